@@ -1,0 +1,225 @@
+/*
+ * sedcol.h -- C ABI of the MI355X sediment-column library (libsedcol.so).
+ *
+ * The library replaces the reference's sediment-column hot path:
+ *   - get_rhs   src/drivers/fabm_sediment_driver.F90:575-717 (with diff3d
+ *               :739-825 and the local reaction rates called at :700),
+ *   - ode_solver src/utilities/solver_library.F90:80-189 (methods 0..3 keep
+ *               their meaning; 4..6 add positivity-preserving EMP steppers),
+ * driven per horizontal column on one GPU.  Fortran callers reach it through
+ * the ISO_C_BINDING module mossco-code_amd/fortran/sedcol_hip.F90.
+ *
+ * Conventions
+ *   - every function returns int status: 0 = SEDCOL_OK, otherwise a
+ *     sedcol_status code; sedcol_last_error() describes the last failure of
+ *     the calling thread.  Nothing calls exit/stop/abort.
+ *   - all pointers are caller-owned host memory, read or written during the
+ *     call only (the library copies); FP64 everywhere.
+ *   - array layout is chosen at create time (sedcol_desc.layout):
+ *       SEDCOL_LAYOUT_FORTRAN  uniform K; cell arrays (ncol, K) and state
+ *                              (ncol, K, nvar) with the column index fastest,
+ *                              i.e. Fortran conc(i,j,k,n) with ncol = I*J;
+ *       SEDCOL_LAYOUT_PACKED   column c owns cells off[c]..off[c+1]-1 (top
+ *                              first, off from nlayers); cell arrays (ncell)
+ *                              and state (nvar, ncell), cell index fastest.
+ *     Per-column arrays are (ncol); per-column-per-variable arrays are
+ *     (nvar, ncol) with the column index fastest, like bdys/fluxes
+ *     (fabm_sediment_component.F90:538-541).
+ *   - a handle owns its device buffers and one HIP stream; it is not
+ *     thread-safe (one handle per host thread, as type_sed is per PET).
+ */
+#ifndef SEDCOL_H
+#define SEDCOL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEDCOL_NVAR 11 /* state variables, order below */
+#define SEDCOL_KMAX 64 /* layers per column */
+#define SEDCOL_NBUD 16 /* budget entries per column */
+
+/* state variable order (all per m3 pore water, mmol) */
+enum sedcol_var {
+  SEDCOL_LDETC = 0, SEDCOL_SDETC, SEDCOL_LDETN, SEDCOL_SDETN, SEDCOL_DETP,
+  SEDCOL_O2, SEDCOL_NO3, SEDCOL_NH4, SEDCOL_ODU, SEDCOL_PO4, SEDCOL_ADSP
+};
+
+/* time-stepping methods.  0..3 = solver_library.F90:32-35. */
+enum sedcol_method {
+  SEDCOL_EULER = 0,
+  SEDCOL_RUNGE_KUTTA_4 = 1,
+  SEDCOL_ADAPTIVE_EULER = 2, /* global x0.25 subcycling, :104-140 */
+  SEDCOL_RUNGE_KUTTA_4_38 = 3,
+  SEDCOL_EMP1 = 4,           /* extended modified Patankar, 1st order */
+  SEDCOL_EMP2 = 5,           /* extended modified Patankar, 2nd order */
+  SEDCOL_EMP_ADAPTIVE = 6    /* EMP-1/EMP-2 pair, per-column step size */
+};
+
+enum sedcol_status {
+  SEDCOL_OK = 0,
+  SEDCOL_EINVAL = 1,   /* bad argument */
+  SEDCOL_EDOMAIN = 2,  /* porosity/grid outside the valid domain
+                          (fabm_sediment_driver.F90:503-530 'stop's) */
+  SEDCOL_ENAN = 3,     /* NaN in the state (fabm_sediment_component.F90:1718-1723) */
+  SEDCOL_EHIP = 4,     /* HIP runtime error */
+  SEDCOL_ELIMIT = 5,   /* adaptive attempt limit reached */
+  SEDCOL_ENOMEM = 6,
+  SEDCOL_EUNSUPPORTED = 7
+};
+
+enum sedcol_layout { SEDCOL_LAYOUT_FORTRAN = 0, SEDCOL_LAYOUT_PACKED = 1 };
+
+typedef struct sedcol_handle sedcol_handle;
+
+/* Domain description, the counterpart of type_rhs_driver's
+ * inum/jnum/knum/nvar/mask (solver_library.F90:37-49) and fabm_sed_grid
+ * (fabm_sediment_driver.F90:41-56). */
+typedef struct {
+  int32_t ncol;            /* columns (I*J) */
+  int32_t kmax;            /* K for a uniform grid, max K otherwise (<= SEDCOL_KMAX) */
+  int32_t nvar;            /* must equal SEDCOL_NVAR */
+  int32_t layout;          /* enum sedcol_layout */
+  const int32_t *nlayers;  /* PACKED: layers per column (ncol); NULL = all kmax */
+  const int32_t *mask;     /* ncol, nonzero = land (never integrated); NULL = all wet */
+} sedcol_desc;
+
+/* Parameters.  Transport keys follow &sed_nml and type_sed
+ * (fabm_sediment_driver.F90:73-89,211-233); the solver keys type_rhs_driver
+ * (solver_library.F90:40-41); kinetic rates are SI (s-1, mmol m-3 s-1). */
+typedef struct {
+  double D0;                  /* molecular diffusivity at 0 degC, cm2 d-1 ('diffusivity') */
+  double dt_min;              /* ADAPTIVE_EULER floor (solver_library.F90:40) */
+  double relative_change_min; /* ADAPTIVE_EULER criterion (:41) */
+  double rL, rS, CprodMax;    /* rLabile, rSemilabile (s-1), CprodMax (mmol m-3 s-1) */
+  double rnit, ksO2nitri, rODUox, ksO2oduox, ksO2oxic;
+  double ksNO3denit, kinO2denit, kinNO3anox, kinO2anox;
+  double rPads, PAdsODU;      /* P adsorption rate (s-1), release half-saturation */
+  double emp_rtol, emp_atol;  /* adaptive EMP error tolerances */
+  double emp_dt_min, emp_dt_max;
+  int32_t bcup_dis;           /* bcup_dissolved_variables: 1 flux, 2 Dirichlet, 3 closed */
+  int32_t bcup_part;          /* bcup_particulate_variables: 1 flux, 3 closed */
+  int32_t bcdown;             /* 1, 2 or 3 (reference get_rhs uses 3) */
+  int32_t clamp;              /* clamp to the minimum 0 after each fixed step, as
+                                 fabm_sediment_component.F90:1726-1731 does */
+} sedcol_params;
+
+/* Per-column diagnostics (sedcol_get_diagnostics), the per-column form of
+ * last_min_dt / last_min_dt_grid_cell (solver_library.F90:42-43,130-135). */
+typedef struct {
+  int64_t accepted;   /* accepted steps */
+  int64_t rejected;   /* rejected adaptive attempts */
+  int64_t forced;     /* accepted at emp_dt_min with error above tolerance */
+  int64_t zero_sink;  /* destruction terms of exhausted components (EMP) */
+  double dt_last;     /* last accepted step */
+  double dt_next;     /* step size carried into the next call */
+  double dt_minacc;   /* smallest accepted step */
+  double p_min;       /* smallest Patankar factor */
+  int32_t status;     /* 0 ok, 1 NaN, 2 attempt limit */
+  int32_t pad;
+} sedcol_coldiag;
+
+/* --- lifecycle ---------------------------------------------------------- */
+
+/* Replaces type_sed allocation in initialize (fabm_sediment_driver.F90:191-368)
+ * and the conc allocation of its callers (fabm_sediment_component.F90:521,
+ * examples/standalone/omexdia_p/main.F90:75-80).  Device 0 of the process. */
+int sedcol_create(const sedcol_desc *desc, sedcol_handle **out);
+int sedcol_destroy(sedcol_handle *h);
+const char *sedcol_last_error(void);
+
+/* --- set-up ------------------------------------------------------------- */
+
+/* Replaces the &sed_nml read and type_sed scalars (fabm_sediment_driver.F90:
+ * 211-250) and the FABM parameter set (fabm_sed.nml:51-77). */
+int sedcol_set_params(sedcol_handle *h, const sedcol_params *p);
+
+/* Replaces init_grid (:127-177), the porosity / bioturbation set-up of
+ * initialize (:278-313) and update_porosity (:393-442): the caller supplies
+ * dz (layer heights), dzc (distance to the layer above, value at the top
+ * layer ignored; the reference's dzc(k-1) is stored at layer k), phi
+ * (porosity), phi_intf (interface porosity), bioturb_factor (at the upper
+ * interface of each layer) and the per-column bioturbation coefficient
+ * (cm2 d-1).  Validates as fabm_sed_check_domain (:488-545) and returns
+ * SEDCOL_EDOMAIN instead of stopping. */
+int sedcol_set_geometry(sedcol_handle *h, const double *dz, const double *dzc,
+                        const double *phi, const double *phi_intf,
+                        const double *bioturb_factor, const double *bioturbation);
+
+/* Replaces bdys(:,:,1) (temperature), bdys(:,:,n+1) (bottom-water
+ * concentrations) and fluxes(:,:,n) (prescribed top fluxes, positive
+ * downward) as assembled by get_boundary_conditions
+ * (fabm_sediment_component.F90:1865-2030).  Forcing stays constant until the
+ * next call. */
+int sedcol_set_forcing(sedcol_handle *h, const double *temp, const double *c_bw,
+                       const double *flux_top);
+
+/* A piecewise-constant forcing series of nrec records, record r applying on
+ * [t0 + r*dt_rec, t0 + (r+1)*dt_rec); arrays are (nrec, ncol),
+ * (nrec, nvar, ncol), (nrec, nvar, ncol).  Used by sedcol_integrate.  No
+ * reference counterpart: the reference re-reads forcing every coupling step
+ * (fabm_sediment_component.F90:1665); this lets one launch span many records. */
+int sedcol_set_forcing_series(sedcol_handle *h, int32_t nrec, double t0, double dt_rec,
+                              const double *temp, const double *c_bw,
+                              const double *flux_top);
+
+/* State in / out (conc pointer association, fabm_sediment_component.F90:521,
+ * :1666-1667; init_concentrations :449-481 is the caller's). */
+int sedcol_set_state(sedcol_handle *h, const double *conc);
+int sedcol_get_state(sedcol_handle *h, double *conc);
+
+/* --- hot path ----------------------------------------------------------- */
+
+/* get_rhs (fabm_sediment_driver.F90:575-717): rhs of conc under the current
+ * forcing; flux_top_out (nvar, ncol, may be NULL) = intFlux(:,:,1) as stored
+ * into fluxes(:,:,n) at :692.  Masked columns return rhs = 0. */
+int sedcol_rhs(sedcol_handle *h, const double *conc, double *rhs, double *flux_top_out);
+
+/* ode_solver(rhs_driver, dt, method) (solver_library.F90:80-189) on the
+ * caller's conc, updated in place (the drop-in call used at
+ * fabm_sediment_component.F90:1715 and main.F90:123).  flux_top_out = top
+ * fluxes of the last right-hand side evaluated (NULL allowed); the clamp of
+ * sedcol_params.clamp is applied after the step, as the callers do. */
+int sedcol_step(sedcol_handle *h, double dt, int32_t method, double *conc_inout,
+                double *flux_top_out);
+
+/* nsteps fixed steps (methods 0..5) of dt on the device-resident state
+ * (sedcol_set_state), steps_per_launch steps per kernel launch (0 = all in
+ * one persistent launch).  Equivalent to nsteps ode_solver calls plus the
+ * caller's clamp, the Run loop of fabm_sediment_component.F90:1700-1769. */
+int sedcol_run(sedcol_handle *h, double dt, int32_t nsteps, int32_t method,
+               int32_t steps_per_launch);
+
+/* Adaptive EMP (method 6) on the device-resident state from t_start to
+ * t_end under the forcing series (or the current forcing), each column with
+ * its own step size; one persistent launch. */
+int sedcol_integrate(sedcol_handle *h, double t_start, double t_end);
+
+/* --- outputs ------------------------------------------------------------ */
+
+int sedcol_get_fluxes(sedcol_handle *h, double *flux_top_out); /* (nvar, ncol) */
+/* budget (SEDCOL_NBUD, ncol) with entries fastest: 0..10 time-integrated top
+ * flux per variable, 11 integrated DIC release, 12 integrated N2 release,
+ * 13..15 mass added by the minimum clamp for C, N, P (mmol m-2). */
+int sedcol_get_budget(sedcol_handle *h, double *budget);
+int sedcol_reset_budget(sedcol_handle *h);
+/* inventory (nvar, ncol): sum_k phi_k dz_k c_k (mmol m-2) of the resident state */
+int sedcol_get_inventory(sedcol_handle *h, double *inventory);
+int sedcol_get_diagnostics(sedcol_handle *h, sedcol_coldiag *diag); /* ncol entries */
+int sedcol_reset_diagnostics(sedcol_handle *h);
+/* steady-state residual per column: max |rhs| / max(|c|, cfloor[n]) (s-1) */
+int sedcol_residual(sedcol_handle *h, const double *cfloor, double *resid);
+/* last_min_dt of the reference ADAPTIVE_EULER (solver_library.F90:42,130-135) */
+int sedcol_get_last_min_dt(sedcol_handle *h, double *last_min_dt);
+int sedcol_synchronize(sedcol_handle *h);
+
+/* timing of the last sedcol_run / sedcol_integrate measured with HIP events
+ * on the handle's stream: total milliseconds and number of kernel launches */
+int sedcol_last_timing(sedcol_handle *h, double *ms, int32_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

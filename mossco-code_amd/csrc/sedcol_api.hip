@@ -1,0 +1,705 @@
+// sedcol_api.hip -- host side of the C ABI declared in include/sedcol.h.
+//
+// A handle owns the device copy of one sediment domain (geometry, forcing,
+// state, accumulators, diagnostics) and one HIP stream.  Every entry point
+// validates its arguments, returns a sedcol_status code and records a message
+// for sedcol_last_error(); nothing aborts the process.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sedcol_kernels.hpp"
+
+using sc::Dev;
+using sc::NV;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+  g_err = msg;
+  return code;
+}
+
+#define HIPOK(call)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(SEDCOL_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+constexpr int kLpc[3] = {16, 32, 64};
+
+}  // namespace
+
+struct sedcol_handle {
+  int ncol = 0, kmax = 0, layout = 0;
+  long long ncell = 0;
+  std::vector<int> off;   // ncol+1
+  std::vector<int> mask;  // ncol
+  std::vector<int> bins[3];
+  int *d_off = nullptr;
+  int *d_list[3] = {nullptr, nullptr, nullptr};
+  double *d_dz = nullptr, *d_dzc = nullptr, *d_phi = nullptr, *d_phiI = nullptr, *d_bfac = nullptr;
+  double *d_biot = nullptr;
+  int nrec = 0;
+  double t0 = 0.0, dt_rec = INFINITY;
+  double *d_temp = nullptr, *d_cbw = nullptr, *d_fup = nullptr;
+  size_t forcing_cap = 0;  // records allocated
+  double *d_conc = nullptr, *d_cellacc = nullptr, *d_colacc = nullptr, *d_flux = nullptr;
+  sedcol_coldiag *d_diag = nullptr;
+  double *d_scr = nullptr;  // NV*ncell scratch (rhs in/out, adaptive Euler rates)
+  double *d_scr2 = nullptr; // NV*ncell scratch
+  double *d_small = nullptr;  // NV*ncol + ncol scratch
+  int *d_flag = nullptr;
+  sedcol_params p{};
+  bool have_params = false, have_geom = false, have_forcing = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int last_launches = 0;
+  double last_min_dt = 1.e20;
+
+  Dev dev() const
+  {
+    Dev d;
+    d.ncol = ncol;
+    d.ncell = ncell;
+    d.off = d_off;
+    d.dz = d_dz;
+    d.dzc = d_dzc;
+    d.phi = d_phi;
+    d.phiI = d_phiI;
+    d.bfac = d_bfac;
+    d.biot = d_biot;
+    d.nrec = nrec;
+    d.t0 = t0;
+    d.dt_rec = dt_rec;
+    d.temp = d_temp;
+    d.cbw = d_cbw;
+    d.fup = d_fup;
+    d.conc = d_conc;
+    d.cellacc = d_cellacc;
+    d.colacc = d_colacc;
+    d.flux = d_flux;
+    d.diag = d_diag;
+    return d;
+  }
+};
+
+namespace {
+
+// Fortran layout index of cell (c, k) and state (c, k, n) for a uniform grid
+inline long long fidx(const sedcol_handle *h, int c, int k) { return (long long)k * h->ncol + c; }
+
+// host cell array (caller layout) -> packed
+void pack_cells(const sedcol_handle *h, const double *src, std::vector<double> &dst)
+{
+  dst.resize(h->ncell);
+  if (h->layout == SEDCOL_LAYOUT_PACKED) {
+    std::memcpy(dst.data(), src, sizeof(double) * h->ncell);
+    return;
+  }
+  for (int c = 0; c < h->ncol; ++c)
+    for (int k = 0; k < h->kmax; ++k) dst[(long long)c * h->kmax + k] = src[fidx(h, c, k)];
+}
+
+void pack_state(const sedcol_handle *h, const double *src, std::vector<double> &dst)
+{
+  dst.resize((size_t)NV * h->ncell);
+  if (h->layout == SEDCOL_LAYOUT_PACKED) {
+    std::memcpy(dst.data(), src, sizeof(double) * NV * h->ncell);
+    return;
+  }
+  const long long plane = (long long)h->ncol * h->kmax;
+  for (int n = 0; n < NV; ++n)
+    for (int k = 0; k < h->kmax; ++k)
+      for (int c = 0; c < h->ncol; ++c)
+        dst[n * h->ncell + (long long)c * h->kmax + k] = src[n * plane + fidx(h, c, k)];
+}
+
+void unpack_state(const sedcol_handle *h, const std::vector<double> &src, double *dst)
+{
+  if (h->layout == SEDCOL_LAYOUT_PACKED) {
+    std::memcpy(dst, src.data(), sizeof(double) * NV * h->ncell);
+    return;
+  }
+  const long long plane = (long long)h->ncol * h->kmax;
+  for (int n = 0; n < NV; ++n)
+    for (int k = 0; k < h->kmax; ++k)
+      for (int c = 0; c < h->ncol; ++c)
+        dst[n * plane + fidx(h, c, k)] = src[n * h->ncell + (long long)c * h->kmax + k];
+}
+
+int check_handle(sedcol_handle *h, bool need_state_inputs)
+{
+  if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  if (need_state_inputs) {
+    if (!h->have_params) return fail(SEDCOL_EINVAL, "sedcol_set_params has not been called");
+    if (!h->have_geom) return fail(SEDCOL_EINVAL, "sedcol_set_geometry has not been called");
+    if (!h->have_forcing) return fail(SEDCOL_EINVAL, "no forcing set (sedcol_set_forcing)");
+  }
+  return SEDCOL_OK;
+}
+
+int alloc_forcing(sedcol_handle *h, int nrec)
+{
+  if ((size_t)nrec <= h->forcing_cap) return SEDCOL_OK;
+  hipFree(h->d_temp);
+  hipFree(h->d_cbw);
+  hipFree(h->d_fup);
+  h->d_temp = h->d_cbw = h->d_fup = nullptr;
+  h->forcing_cap = 0;
+  HIPOK(hipMalloc(&h->d_temp, sizeof(double) * (size_t)nrec * h->ncol));
+  HIPOK(hipMalloc(&h->d_cbw, sizeof(double) * (size_t)nrec * NV * h->ncol));
+  HIPOK(hipMalloc(&h->d_fup, sizeof(double) * (size_t)nrec * NV * h->ncol));
+  h->forcing_cap = (size_t)nrec;
+  return SEDCOL_OK;
+}
+
+int reset_diag(sedcol_handle *h)
+{
+  std::vector<sedcol_coldiag> dg(h->ncol);
+  for (auto &x : dg) {
+    std::memset(&x, 0, sizeof x);
+    x.dt_minacc = INFINITY;
+    x.p_min = 1.0;
+  }
+  HIPOK(hipMemcpyAsync(h->d_diag, dg.data(), sizeof(sedcol_coldiag) * h->ncol, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int read_flag(sedcol_handle *h, int *flag)
+{
+  HIPOK(hipMemcpyAsync(flag, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int flag_status(int flag)
+{
+  if (flag & 1) return fail(SEDCOL_ENAN, "NaN detected in the state");
+  if (flag & 2) return fail(SEDCOL_ELIMIT, "adaptive attempt limit reached");
+  return SEDCOL_OK;
+}
+
+// forcing record for time t (piecewise constant; first/last extend outward)
+int record_of(const sedcol_handle *h, double t)
+{
+  if (h->nrec <= 1) return 0;
+  double q = std::floor((t - h->t0) / h->dt_rec);
+  if (!(q > 0.0)) return 0;
+  if (q >= h->nrec - 1) return h->nrec - 1;
+  return (int)q;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sedcol_last_error(void) { return g_err.c_str(); }
+
+int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
+{
+  if (!desc || !out) return fail(SEDCOL_EINVAL, "null argument");
+  *out = nullptr;
+  if (desc->nvar != NV) return fail(SEDCOL_EINVAL, "nvar must be SEDCOL_NVAR (11)");
+  if (desc->ncol < 1) return fail(SEDCOL_EINVAL, "ncol must be positive");
+  if (desc->kmax < 1 || desc->kmax > SEDCOL_KMAX) return fail(SEDCOL_EINVAL, "kmax outside 1..SEDCOL_KMAX");
+  if (desc->layout != SEDCOL_LAYOUT_FORTRAN && desc->layout != SEDCOL_LAYOUT_PACKED)
+    return fail(SEDCOL_EINVAL, "unknown layout");
+  if (desc->layout == SEDCOL_LAYOUT_FORTRAN && desc->nlayers)
+    return fail(SEDCOL_EINVAL, "the Fortran layout needs a uniform number of layers (nlayers = NULL)");
+  sedcol_handle *h = new sedcol_handle();
+  h->ncol = desc->ncol;
+  h->kmax = desc->kmax;
+  h->layout = desc->layout;
+  h->off.resize(h->ncol + 1);
+  h->mask.assign(h->ncol, 0);
+  h->off[0] = 0;
+  for (int c = 0; c < h->ncol; ++c) {
+    int K = desc->nlayers ? desc->nlayers[c] : desc->kmax;
+    if (K < 1 || K > desc->kmax) {
+      delete h;
+      return fail(SEDCOL_EINVAL, "nlayers outside 1..kmax");
+    }
+    h->off[c + 1] = h->off[c] + K;
+    if (desc->mask) h->mask[c] = desc->mask[c] != 0;
+  }
+  h->ncell = h->off[h->ncol];
+  for (int c = 0; c < h->ncol; ++c) {
+    if (h->mask[c]) continue;
+    int K = h->off[c + 1] - h->off[c];
+    h->bins[K <= 16 ? 0 : K <= 32 ? 1 : 2].push_back(c);
+  }
+  auto cleanup = [&](int code) {
+    sedcol_destroy(h);
+    return code;
+  };
+#define HIPOK_C(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return cleanup(fail(SEDCOL_EHIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
+  } while (0)
+  HIPOK_C(hipSetDevice(0));
+  HIPOK_C(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIPOK_C(hipEventCreate(&h->ev0));
+  HIPOK_C(hipEventCreate(&h->ev1));
+  const size_t cb = sizeof(double) * (size_t)h->ncell;
+  HIPOK_C(hipMalloc(&h->d_off, sizeof(int) * (h->ncol + 1)));
+  HIPOK_C(hipMemcpy(h->d_off, h->off.data(), sizeof(int) * (h->ncol + 1), hipMemcpyHostToDevice));
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty()) {
+      HIPOK_C(hipMalloc(&h->d_list[b], sizeof(int) * h->bins[b].size()));
+      HIPOK_C(hipMemcpy(h->d_list[b], h->bins[b].data(), sizeof(int) * h->bins[b].size(), hipMemcpyHostToDevice));
+    }
+  HIPOK_C(hipMalloc(&h->d_dz, cb));
+  HIPOK_C(hipMalloc(&h->d_dzc, cb));
+  HIPOK_C(hipMalloc(&h->d_phi, cb));
+  HIPOK_C(hipMalloc(&h->d_phiI, cb));
+  HIPOK_C(hipMalloc(&h->d_bfac, cb));
+  HIPOK_C(hipMalloc(&h->d_biot, sizeof(double) * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_conc, NV * cb));
+  HIPOK_C(hipMalloc(&h->d_cellacc, 5 * cb));
+  HIPOK_C(hipMalloc(&h->d_colacc, sizeof(double) * NV * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_flux, sizeof(double) * NV * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_diag, sizeof(sedcol_coldiag) * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_small, sizeof(double) * (NV + 1) * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_flag, sizeof(int)));
+  HIPOK_C(hipMemset(h->d_conc, 0, NV * cb));
+  HIPOK_C(hipMemset(h->d_cellacc, 0, 5 * cb));
+  HIPOK_C(hipMemset(h->d_colacc, 0, sizeof(double) * NV * h->ncol));
+  HIPOK_C(hipMemset(h->d_flux, 0, sizeof(double) * NV * h->ncol));
+#undef HIPOK_C
+  int rc = reset_diag(h);
+  if (rc) return cleanup(rc);
+  *out = h;
+  return SEDCOL_OK;
+}
+
+int sedcol_destroy(sedcol_handle *h)
+{
+  if (!h) return SEDCOL_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  void *ptrs[] = {h->d_off, h->d_list[0], h->d_list[1], h->d_list[2], h->d_dz, h->d_dzc, h->d_phi, h->d_phiI,
+                  h->d_bfac, h->d_biot, h->d_temp, h->d_cbw, h->d_fup, h->d_conc, h->d_cellacc, h->d_colacc,
+                  h->d_flux, h->d_diag, h->d_scr, h->d_scr2, h->d_small, h->d_flag};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return SEDCOL_OK;
+}
+
+int sedcol_set_params(sedcol_handle *h, const sedcol_params *p)
+{
+  if (!h || !p) return fail(SEDCOL_EINVAL, "null argument");
+  if (!(p->bcup_dis >= 1 && p->bcup_dis <= 3)) return fail(SEDCOL_EINVAL, "bcup_dis must be 1, 2 or 3");
+  if (p->bcup_part == 4)
+    return fail(SEDCOL_EUNSUPPORTED, "bcup_part = 4 (distributed POM flux) is not implemented on the GPU");
+  if (!(p->bcup_part == 1 || p->bcup_part == 2 || p->bcup_part == 3))
+    return fail(SEDCOL_EINVAL, "bcup_part must be 1, 2 or 3");
+  if (!(p->bcdown >= 1 && p->bcdown <= 3)) return fail(SEDCOL_EINVAL, "bcdown must be 1, 2 or 3");
+  if (!(p->emp_dt_min > 0.0) || !(p->emp_dt_max >= p->emp_dt_min))
+    return fail(SEDCOL_EINVAL, "need 0 < emp_dt_min <= emp_dt_max");
+  if (!(p->emp_rtol >= 0.0) || !(p->emp_atol > 0.0)) return fail(SEDCOL_EINVAL, "need emp_rtol >= 0, emp_atol > 0");
+  h->p = *p;
+  h->have_params = true;
+  return SEDCOL_OK;
+}
+
+int sedcol_set_geometry(sedcol_handle *h, const double *dz, const double *dzc, const double *phi,
+                        const double *phi_intf, const double *bioturb_factor, const double *bioturbation)
+{
+  if (!h || !dz || !dzc || !phi || !phi_intf || !bioturb_factor || !bioturbation)
+    return fail(SEDCOL_EINVAL, "null argument");
+  std::vector<double> a[5];
+  pack_cells(h, dz, a[0]);
+  pack_cells(h, dzc, a[1]);
+  pack_cells(h, phi, a[2]);
+  pack_cells(h, phi_intf, a[3]);
+  pack_cells(h, bioturb_factor, a[4]);
+  // fabm_sed_check_domain (fabm_sediment_driver.F90:488-545) for wet columns;
+  // porosity 1 is excluded too: particulate divergence divides by 1 - phi
+  for (int c = 0; c < h->ncol; ++c) {
+    if (h->mask[c]) continue;
+    if (!(bioturbation[c] >= 0.0) || !std::isfinite(bioturbation[c]))
+      return fail(SEDCOL_EDOMAIN, "bioturbation coefficient must be finite and >= 0 (column " + std::to_string(c) + ")");
+    for (int i = h->off[c]; i < h->off[c + 1]; ++i) {
+      const int k = i - h->off[c];
+      if (!(a[2][i] > 0.0 && a[2][i] < 1.0))
+        return fail(SEDCOL_EDOMAIN, "sediment porosity outside (0,1) at column " + std::to_string(c) + " layer " + std::to_string(k));
+      if (!(a[3][i] > 0.0 && a[3][i] <= 1.0))
+        return fail(SEDCOL_EDOMAIN, "interface porosity outside (0,1] at column " + std::to_string(c));
+      if (!(a[0][i] > 0.0) || !std::isfinite(a[0][i]))
+        return fail(SEDCOL_EDOMAIN, "sediment layer height <= 0 at column " + std::to_string(c) + " layer " + std::to_string(k));
+      if (k > 0 && (!(a[1][i] > 0.0) || !std::isfinite(a[1][i])))
+        return fail(SEDCOL_EDOMAIN, "sediment central layer difference <= 0 at column " + std::to_string(c) + " layer " + std::to_string(k));
+      if (!(a[4][i] >= 0.0) || !std::isfinite(a[4][i]))
+        return fail(SEDCOL_EDOMAIN, "bioturbation factor must be finite and >= 0 at column " + std::to_string(c));
+    }
+  }
+  const size_t cb = sizeof(double) * (size_t)h->ncell;
+  HIPOK(hipMemcpyAsync(h->d_dz, a[0].data(), cb, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_dzc, a[1].data(), cb, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_phi, a[2].data(), cb, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_phiI, a[3].data(), cb, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_bfac, a[4].data(), cb, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_biot, bioturbation, sizeof(double) * h->ncol, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  h->have_geom = true;
+  return SEDCOL_OK;
+}
+
+int sedcol_set_forcing_series(sedcol_handle *h, int32_t nrec, double t0, double dt_rec, const double *temp,
+                              const double *c_bw, const double *flux_top)
+{
+  if (!h || !temp || !c_bw || !flux_top) return fail(SEDCOL_EINVAL, "null argument");
+  if (nrec < 1) return fail(SEDCOL_EINVAL, "nrec must be positive");
+  if (nrec > 1 && !(dt_rec > 0.0 && std::isfinite(dt_rec))) return fail(SEDCOL_EINVAL, "dt_rec must be positive");
+  if (!std::isfinite(t0)) return fail(SEDCOL_EINVAL, "t0 must be finite");
+  int rc = alloc_forcing(h, nrec);
+  if (rc) return rc;
+  HIPOK(hipMemcpyAsync(h->d_temp, temp, sizeof(double) * (size_t)nrec * h->ncol, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_cbw, c_bw, sizeof(double) * (size_t)nrec * NV * h->ncol, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_fup, flux_top, sizeof(double) * (size_t)nrec * NV * h->ncol, hipMemcpyHostToDevice,
+                       h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  h->nrec = nrec;
+  h->t0 = t0;
+  h->dt_rec = nrec > 1 ? dt_rec : INFINITY;
+  h->have_forcing = true;
+  return SEDCOL_OK;
+}
+
+int sedcol_set_forcing(sedcol_handle *h, const double *temp, const double *c_bw, const double *flux_top)
+{
+  return sedcol_set_forcing_series(h, 1, 0.0, INFINITY, temp, c_bw, flux_top);
+}
+
+int sedcol_set_state(sedcol_handle *h, const double *conc)
+{
+  if (!h || !conc) return fail(SEDCOL_EINVAL, "null argument");
+  std::vector<double> buf;
+  pack_state(h, conc, buf);
+  HIPOK(hipMemcpyAsync(h->d_conc, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int sedcol_get_state(sedcol_handle *h, double *conc)
+{
+  if (!h || !conc) return fail(SEDCOL_EINVAL, "null argument");
+  std::vector<double> buf((size_t)NV * h->ncell);
+  HIPOK(hipMemcpyAsync(buf.data(), h->d_conc, sizeof(double) * buf.size(), hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  unpack_state(h, buf, conc);
+  return SEDCOL_OK;
+}
+
+static int ensure_scratch(sedcol_handle *h)
+{
+  if (!h->d_scr) HIPOK(hipMalloc(&h->d_scr, sizeof(double) * NV * (size_t)h->ncell));
+  if (!h->d_scr2) HIPOK(hipMalloc(&h->d_scr2, sizeof(double) * NV * (size_t)h->ncell));
+  return SEDCOL_OK;
+}
+
+int sedcol_rhs(sedcol_handle *h, const double *conc, double *rhs, double *flux_top_out)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!conc || !rhs) return fail(SEDCOL_EINVAL, "null argument");
+  if ((rc = ensure_scratch(h))) return rc;
+  std::vector<double> buf;
+  pack_state(h, conc, buf);
+  HIPOK(hipMemcpyAsync(h->d_scr, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemsetAsync(h->d_scr2, 0, sizeof(double) * buf.size(), h->stream));
+  HIPOK(hipMemsetAsync(h->d_small, 0, sizeof(double) * NV * h->ncol, h->stream));
+  const Dev d = h->dev();
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty())
+      sc::launch_rhs(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, 0, h->d_scr, h->d_scr2,
+                     h->d_small);
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(buf.data(), h->d_scr2, sizeof(double) * buf.size(), hipMemcpyDeviceToHost, h->stream));
+  std::vector<double> fl((size_t)NV * h->ncol);
+  HIPOK(hipMemcpyAsync(fl.data(), h->d_small, sizeof(double) * fl.size(), hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  unpack_state(h, buf, rhs);
+  if (flux_top_out) std::memcpy(flux_top_out, fl.data(), sizeof(double) * fl.size());
+  return SEDCOL_OK;
+}
+
+// ADAPTIVE_EULER over dt on the resident state (solver_library.F90:104-140)
+static int adaptive_euler(sedcol_handle *h, double dt, int rec, int *launches)
+{
+  int rc = ensure_scratch(h);
+  if (rc) return rc;
+  const Dev d = h->dev();
+  double dt_int = 0.0, dt_red = dt;
+  bool need_rhs = true;
+  while (dt_int < dt) {
+    if (need_rhs) {
+      for (int b = 0; b < 3; ++b)
+        if (!h->bins[b].empty()) {
+          sc::launch_rhs_store(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, rec, h->d_scr,
+                               h->d_scr2);
+          ++*launches;
+        }
+      need_rhs = false;
+    }
+    HIPOK(hipMemsetAsync(h->d_flag, 0, sizeof(int), h->stream));
+    for (int b = 0; b < 3; ++b)
+      if (!h->bins[b].empty()) {
+        sc::launch_ae_check(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), dt_red,
+                            h->p.relative_change_min, h->d_scr, h->d_flag);
+        ++*launches;
+      }
+    HIPOK(hipGetLastError());
+    int viol = 0;
+    if ((rc = read_flag(h, &viol))) return rc;
+    if (viol && dt_red > h->p.dt_min) {
+      dt_red = dt_red * 0.25;  // the state is unchanged: the right-hand side is reused
+    } else {
+      if (dt_red < h->last_min_dt) h->last_min_dt = dt_red;
+      for (int b = 0; b < 3; ++b)
+        if (!h->bins[b].empty()) {
+          sc::launch_ae_commit(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), dt_red, h->d_scr,
+                               h->d_scr2);
+          ++*launches;
+        }
+      dt_int = dt_int + dt_red;
+      need_rhs = true;
+    }
+  }
+  HIPOK(hipGetLastError());
+  return SEDCOL_OK;
+}
+
+static int run_fixed(sedcol_handle *h, double dt, int nsteps, int method, int steps_per_launch, int *launches)
+{
+  const Dev d = h->dev();
+  const int rec = 0;
+  int left = nsteps;
+  const int chunk = steps_per_launch > 0 ? steps_per_launch : nsteps;
+  while (left > 0) {
+    const int ns = std::min(left, chunk);
+    for (int b = 0; b < 3; ++b)
+      if (!h->bins[b].empty()) {
+        sc::launch_fixed(kLpc[b], method, h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, rec, dt, ns,
+                         h->d_flag);
+        ++*launches;
+      }
+    HIPOK(hipGetLastError());
+    left -= ns;
+  }
+  return SEDCOL_OK;
+}
+
+int sedcol_run(sedcol_handle *h, double dt, int32_t nsteps, int32_t method, int32_t steps_per_launch)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(SEDCOL_EINVAL, "nsteps must be >= 0");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(SEDCOL_EINVAL, "dt must be positive");
+  if (method < SEDCOL_EULER || method > SEDCOL_EMP2)
+    return fail(SEDCOL_EINVAL, "sedcol_run takes methods 0..5 (use sedcol_integrate for method 6)");
+  if (h->nrec > 1) return fail(SEDCOL_EINVAL, "fixed-step runs use a single forcing record (sedcol_set_forcing)");
+  HIPOK(hipMemsetAsync(h->d_flag, 0, sizeof(int), h->stream));
+  int launches = 0;
+  HIPOK(hipEventRecord(h->ev0, h->stream));
+  if (method == SEDCOL_ADAPTIVE_EULER) {
+    for (int s = 0; s < nsteps; ++s) {
+      if ((rc = adaptive_euler(h, dt, 0, &launches))) return rc;
+      if (h->p.clamp) {
+        const Dev d = h->dev();
+        for (int b = 0; b < 3; ++b)
+          if (!h->bins[b].empty()) {
+            sc::launch_clamp(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->d_flag);
+            ++launches;
+          }
+      }
+    }
+  } else {
+    if ((rc = run_fixed(h, dt, nsteps, method, steps_per_launch, &launches))) return rc;
+  }
+  HIPOK(hipEventRecord(h->ev1, h->stream));
+  HIPOK(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPOK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->last_launches = launches;
+  int flag = 0;
+  if ((rc = read_flag(h, &flag))) return rc;
+  return flag_status(flag);
+}
+
+int sedcol_integrate(sedcol_handle *h, double t_start, double t_end)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!std::isfinite(t_start) || !std::isfinite(t_end) || !(t_end >= t_start))
+    return fail(SEDCOL_EINVAL, "need finite t_start <= t_end");
+  const int r0 = record_of(h, t_start), r1 = record_of(h, t_end);
+  HIPOK(hipMemsetAsync(h->d_flag, 0, sizeof(int), h->stream));
+  const Dev d = h->dev();
+  int launches = 0;
+  const long long max_attempts = 1ll << 40;
+  HIPOK(hipEventRecord(h->ev0, h->stream));
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty()) {
+      sc::launch_emp_adaptive(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, t_start, t_end, r0,
+                              r1, max_attempts, h->d_flag);
+      ++launches;
+    }
+  HIPOK(hipGetLastError());
+  HIPOK(hipEventRecord(h->ev1, h->stream));
+  HIPOK(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPOK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->last_launches = launches;
+  int flag = 0;
+  if ((rc = read_flag(h, &flag))) return rc;
+  return flag_status(flag);
+}
+
+int sedcol_step(sedcol_handle *h, double dt, int32_t method, double *conc_inout, double *flux_top_out)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!conc_inout) return fail(SEDCOL_EINVAL, "null argument");
+  if (method < SEDCOL_EULER || method > SEDCOL_EMP_ADAPTIVE) return fail(SEDCOL_EINVAL, "unknown method");
+  if ((rc = sedcol_set_state(h, conc_inout))) return rc;
+  if (method == SEDCOL_EMP_ADAPTIVE) {
+    if (h->nrec > 1) return fail(SEDCOL_EINVAL, "sedcol_step uses a single forcing record");
+    rc = sedcol_integrate(h, 0.0, dt);
+  } else {
+    rc = sedcol_run(h, dt, 1, method, 0);
+  }
+  int rc2 = sedcol_get_state(h, conc_inout);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (flux_top_out) return sedcol_get_fluxes(h, flux_top_out);
+  return SEDCOL_OK;
+}
+
+int sedcol_get_fluxes(sedcol_handle *h, double *flux_top_out)
+{
+  if (!h || !flux_top_out) return fail(SEDCOL_EINVAL, "null argument");
+  HIPOK(hipMemcpyAsync(flux_top_out, h->d_flux, sizeof(double) * NV * h->ncol, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int sedcol_get_budget(sedcol_handle *h, double *budget)
+{
+  if (!h || !budget) return fail(SEDCOL_EINVAL, "null argument");
+  double *d_b = nullptr;
+  HIPOK(hipMalloc(&d_b, sizeof(double) * SEDCOL_NBUD * h->ncol));
+  hipError_t e = hipMemsetAsync(d_b, 0, sizeof(double) * SEDCOL_NBUD * h->ncol, h->stream);
+  const Dev d = h->dev();
+  for (int b = 0; b < 3 && e == hipSuccess; ++b)
+    if (!h->bins[b].empty()) sc::launch_budget(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), d_b);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(budget, d_b, sizeof(double) * SEDCOL_NBUD * h->ncol, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(d_b);
+  if (e != hipSuccess) return fail(SEDCOL_EHIP, std::string("sedcol_get_budget: ") + hipGetErrorString(e));
+  return SEDCOL_OK;
+}
+
+int sedcol_reset_budget(sedcol_handle *h)
+{
+  if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  HIPOK(hipMemsetAsync(h->d_cellacc, 0, sizeof(double) * 5 * (size_t)h->ncell, h->stream));
+  HIPOK(hipMemsetAsync(h->d_colacc, 0, sizeof(double) * NV * h->ncol, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int sedcol_get_inventory(sedcol_handle *h, double *inventory)
+{
+  if (!h || !inventory) return fail(SEDCOL_EINVAL, "null argument");
+  HIPOK(hipMemsetAsync(h->d_small, 0, sizeof(double) * NV * h->ncol, h->stream));
+  const Dev d = h->dev();
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty())
+      sc::launch_inventory(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->d_small);
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(inventory, h->d_small, sizeof(double) * NV * h->ncol, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int sedcol_get_diagnostics(sedcol_handle *h, sedcol_coldiag *diag)
+{
+  if (!h || !diag) return fail(SEDCOL_EINVAL, "null argument");
+  HIPOK(hipMemcpyAsync(diag, h->d_diag, sizeof(sedcol_coldiag) * h->ncol, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int sedcol_reset_diagnostics(sedcol_handle *h)
+{
+  if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  h->last_min_dt = 1.e20;
+  return reset_diag(h);
+}
+
+int sedcol_residual(sedcol_handle *h, const double *cfloor, double *resid)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!cfloor || !resid) return fail(SEDCOL_EINVAL, "null argument");
+  double *d_cf = h->d_small + (size_t)NV * h->ncol;  // ncol >= 1 slots; NV needed
+  double *d_res = h->d_small;
+  if (h->ncol < NV) {
+    if ((rc = ensure_scratch(h))) return rc;
+    d_cf = h->d_scr2;
+  }
+  HIPOK(hipMemcpyAsync(d_cf, cfloor, sizeof(double) * NV, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemsetAsync(d_res, 0, sizeof(double) * h->ncol, h->stream));
+  const Dev d = h->dev();
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty())
+      sc::launch_residual(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, 0, d_cf, d_res);
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(resid, d_res, sizeof(double) * h->ncol, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int sedcol_get_last_min_dt(sedcol_handle *h, double *last_min_dt)
+{
+  if (!h || !last_min_dt) return fail(SEDCOL_EINVAL, "null argument");
+  *last_min_dt = h->last_min_dt;
+  return SEDCOL_OK;
+}
+
+int sedcol_synchronize(sedcol_handle *h)
+{
+  if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int sedcol_last_timing(sedcol_handle *h, double *ms, int32_t *launches)
+{
+  if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  if (ms) *ms = h->last_ms;
+  if (launches) *launches = h->last_launches;
+  return SEDCOL_OK;
+}
+
+}  // extern "C"

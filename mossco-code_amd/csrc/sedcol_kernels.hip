@@ -1,0 +1,748 @@
+// sedcol_kernels.hip -- gfx950 kernels of the sediment-column hot path.
+//
+// One column per lane group (LPC lanes, lane = layer, all 11 state
+// variables in VGPRs); a 256-thread block holds 256/LPC columns.  Wet columns
+// are binned by K into LPC = 16/32/64 lists so a group never wastes more than
+// half its lanes.  Fixed-step methods keep the state in registers for a
+// whole window of steps; the adaptive EMP kernel keeps it for the whole
+// integration span and every forcing record inside it.
+#include "sedcol_kernels.hpp"
+
+namespace sc {
+
+constexpr int BLOCK = 256;
+
+// ---------------------------------------------------------------------------
+// per-lane set-up
+// ---------------------------------------------------------------------------
+
+struct Geo {
+  double phiI, bfac, B;
+};
+
+template <int LPC>
+__device__ __forceinline__ void setup_lane(const Dev &d, int c, int l, Lane &L, Geo &G, long long &cell)
+{
+  const int o = d.off[c];
+  L.l = l;
+  L.K = d.off[c + 1] - o;
+  L.active = l < L.K;
+  cell = (long long)o + l;
+  L.dz = L.active ? d.dz[cell] : 1.0;
+  L.dzc = (L.active && l > 0) ? d.dzc[cell] : 1.0;
+  L.phi = L.active ? d.phi[cell] : 0.5;
+  G.phiI = L.active ? d.phiI[cell] : 0.5;
+  G.bfac = L.active ? d.bfac[cell] : 0.0;
+  G.B = d.biot[c];
+}
+
+// load forcing record r of column c: bottom-water concentrations and top
+// fluxes to the group's LDS slot, temperature-dependent diffusivities to L
+template <int LPC>
+__device__ __forceinline__ void load_forcing(const Dev &d, const sedcol_params &p, int c, int r, const Geo &G,
+                                             Lane &L, double *sbc)
+{
+  if (L.l < NV) {
+    sbc[L.l] = d.cbw[((long long)r * NV + L.l) * d.ncol + c];
+    sbc[NV + L.l] = d.fup[((long long)r * NV + L.l) * d.ncol + c];
+  }
+  const double T = d.temp[(long long)r * d.ncol + c];
+  diffusivities(p, G.B, T, G.phiI, G.bfac, L);
+}
+
+template <int LPC>
+__device__ __forceinline__ bool group_any(bool b)
+{
+  return lg_max<LPC>(b ? 1.0 : 0.0) > 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// single right-hand side (get_rhs) for parity and diagnostics
+// ---------------------------------------------------------------------------
+
+template <int LPC>
+__global__ __launch_bounds__(BLOCK) void k_rhs(Dev d, const int *list, int nlist, sedcol_params p, int rec,
+                                               const double *conc, double *rhs, double *flux)
+{
+  constexpr int G = BLOCK / LPC;
+  __shared__ double sbc[G][2 * NV];
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  const bool valid = slot < nlist;
+  const int c = valid ? list[slot] : 0;
+  Lane L;
+  Geo Gm;
+  long long cell = 0;
+  if (valid) {
+    setup_lane<LPC>(d, c, l, L, Gm, cell);
+    load_forcing<LPC>(d, p, c, rec, Gm, L, sbc[g]);
+  }
+  __syncthreads();
+  if (!valid) return;
+  double x[NV];
+#pragma unroll
+  for (int n = 0; n < NV; ++n) x[n] = L.active ? conc[n * d.ncell + cell] : 0.0;
+  double sC, sN;
+  column_rhs<LPC>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
+    if (L.active) rhs[n * d.ncell + cell] = r;
+    if (l == 0 && flux) flux[(long long)n * d.ncol + c] = F;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// fixed-step window: EULER, RK4, RK4 3/8, EMP-1, EMP-2
+// ---------------------------------------------------------------------------
+
+template <int LPC, int METHOD>
+__global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nlist, sedcol_params p, int rec,
+                                                 double dt, int nsteps, int *errflag)
+{
+  constexpr int G = BLOCK / LPC;
+  __shared__ double sbc[G][2 * NV];
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  const bool valid = slot < nlist;
+  const int c = valid ? list[slot] : 0;
+  Lane L;
+  Geo Gm;
+  long long cell = 0;
+  if (valid) {
+    setup_lane<LPC>(d, c, l, L, Gm, cell);
+    load_forcing<LPC>(d, p, c, rec, Gm, L, sbc[g]);
+  }
+  __syncthreads();
+  if (!valid) return;
+  const double *bc = sbc[g];
+  const bool act = L.active;
+
+  double x[NV], fint[NV];
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    x[n] = act ? d.conc[n * d.ncell + cell] : 0.0;
+    fint[n] = l == 0 ? d.colacc[(long long)n * d.ncol + c] : 0.0;
+  }
+  double acc[5];
+#pragma unroll
+  for (int a = 0; a < 5; ++a) acc[a] = act ? d.cellacc[a * d.ncell + cell] : 0.0;
+  sedcol_coldiag dg = d.diag[c];
+  int64_t zs = 0;
+  const double third = 1.0 / 3.0;
+
+  int s = 0;
+  for (; s < nsteps; ++s) {
+    const bool last = s == nsteps - 1;
+    double fc[NV];
+    double sC, sN;
+    if (METHOD == SEDCOL_EULER) {
+      double r0[NV];
+      column_rhs<LPC>(p, L, bc, x, sC, sN, [&](int n, double r, double F) {
+        r0[n] = r;
+        fc[n] = F;
+        if (last && l == 0) d.flux[(long long)n * d.ncol + c] = F;
+      });
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        x[n] = x[n] + dt * r0[n];
+        fint[n] += dt * fc[n];
+      }
+      acc[0] += dt * sC;
+      acc[1] += dt * sN;
+    } else if (METHOD == SEDCOL_RUNGE_KUTTA_4) {
+      // solver_library.F90:142-163
+      double cs[NV], k[NV];
+      double aC, aN;
+      column_rhs<LPC>(p, L, bc, x, sC, sN, [&](int n, double r, double F) {
+        k[n] = 0.5 * r;
+        cs[n] = x[n] + (0.5 * dt) * r;
+        fc[n] = 0.5 * F;
+      });
+      aC = 0.5 * sC;
+      aN = 0.5 * sN;
+      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+        k[n] = k[n] + r;
+        cs[n] = x[n] + (0.5 * dt) * r;
+        fc[n] = fc[n] + F;
+      });
+      aC = aC + sC;
+      aN = aN + sN;
+      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+        k[n] = k[n] + r;
+        cs[n] = x[n] + dt * r;
+        fc[n] = fc[n] + F;
+      });
+      aC = aC + sC;
+      aN = aN + sN;
+      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+        k[n] = k[n] + 0.5 * r;
+        fc[n] = fc[n] + 0.5 * F;
+        if (last && l == 0) d.flux[(long long)n * d.ncol + c] = F;
+      });
+      aC = aC + 0.5 * sC;
+      aN = aN + 0.5 * sN;
+      const double w = dt * third;
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        x[n] = x[n] + w * k[n];
+        fint[n] += w * fc[n];
+      }
+      acc[0] += w * aC;
+      acc[1] += w * aN;
+    } else if (METHOD == SEDCOL_RUNGE_KUTTA_4_38) {
+      // solver_library.F90:164-185
+      double cs[NV], k1[NV], k2[NV];
+      double f1[NV], f2[NV], s1C, s1N, s2C, s2N;
+      column_rhs<LPC>(p, L, bc, x, sC, sN, [&](int n, double r, double F) {
+        k1[n] = r;
+        cs[n] = x[n] + (third * dt) * r;
+        f1[n] = F;
+      });
+      s1C = sC;
+      s1N = sN;
+      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+        k2[n] = r;
+        cs[n] = x[n] + dt * (r - third * k1[n]);
+        f2[n] = F;
+      });
+      s2C = sC;
+      s2N = sN;
+      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+        cs[n] = x[n] + dt * ((k1[n] - k2[n]) + r);
+        k1[n] = (k1[n] + 3.0 * k2[n]) + 3.0 * r;
+        fc[n] = (f1[n] + 3.0 * f2[n]) + 3.0 * F;
+      });
+      double aC = (s1C + 3.0 * s2C) + 3.0 * sC;
+      double aN = (s1N + 3.0 * s2N) + 3.0 * sN;
+      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+        k1[n] = k1[n] + r;
+        fc[n] = fc[n] + F;
+        if (last && l == 0) d.flux[(long long)n * d.ncol + c] = F;
+      });
+      aC = aC + sC;
+      aN = aN + sN;
+      const double w = (dt * 1.0) / 8.0;
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        x[n] = x[n] + w * k1[n];
+        fint[n] += w * fc[n];
+      }
+      acc[0] += w * aC;
+      acc[1] += w * aN;
+    } else {  // EMP-1 / EMP-2 with a fixed step
+      double f0[NV];
+      column_rhs<LPC>(p, L, bc, x, sC, sN, [&](int n, double r, double F) {
+        f0[n] = r;
+        fc[n] = F;
+        if (METHOD == SEDCOL_EMP1 && last && l == 0) d.flux[(long long)n * d.ncol + c] = F;
+      });
+      const double p1 = psolve<LPC>(x, f0, dt, act, zs);
+      if (p1 < dg.p_min) dg.p_min = p1;
+      if (METHOD == SEDCOL_EMP1) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+          x[n] = x[n] + (dt * p1) * f0[n];
+          fint[n] += (dt * p1) * fc[n];
+        }
+        acc[0] += (dt * p1) * sC;
+        acc[1] += (dt * p1) * sN;
+      } else {
+        double c1[NV];
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+          const double y = x[n] + (dt * p1) * f0[n];
+          c1[n] = y < 0.0 ? 0.0 : y;
+        }
+        const double s0C = sC, s0N = sN;
+        column_rhs<LPC>(p, L, bc, c1, sC, sN, [&](int n, double r, double F) {
+          f0[n] = (f0[n] + r) * 0.5;
+          fc[n] = (fc[n] + F) * 0.5;
+          if (last && l == 0) d.flux[(long long)n * d.ncol + c] = F;
+        });
+        const double p2 = psolve<LPC>(x, f0, dt, act, zs);
+        if (p2 < dg.p_min) dg.p_min = p2;
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+          x[n] = x[n] + (dt * p2) * f0[n];
+          fint[n] += (dt * p2) * fc[n];
+        }
+        acc[0] += (dt * p2) * ((s0C + sC) * 0.5);
+        acc[1] += (dt * p2) * ((s0N + sN) * 0.5);
+      }
+    }
+    // clamp: Patankar rounding guard (EMP), or the callers' minimum clamp
+    if (METHOD >= SEDCOL_EMP1 || p.clamp) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n)
+        if (x[n] < 0.0) {
+          if (elem(n) >= 0) acc[2 + elem(n)] += 0.0 - x[n];
+          x[n] = 0.0;
+        }
+    }
+    bool nan = false;
+#pragma unroll
+    for (int n = 0; n < NV; ++n) nan |= act && (x[n] != x[n]);
+    dg.accepted += 1;
+    dg.dt_last = dt;
+    if (dt < dg.dt_minacc) dg.dt_minacc = dt;
+    if (group_any<LPC>(nan)) {
+      dg.status = 1;
+      if (l == 0) atomicOr(errflag, 1);
+      break;
+    }
+  }
+
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    if (act) d.conc[n * d.ncell + cell] = x[n];
+    if (l == 0) d.colacc[(long long)n * d.ncol + c] = fint[n];
+  }
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+    if (act) d.cellacc[a * d.ncell + cell] = acc[a];
+  const double zsum = lg_sum<LPC>((double)zs);
+  if (l == 0) {
+    dg.zero_sink += (int64_t)zsum;
+    d.diag[c] = dg;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adaptive EMP: per-column step size, persistent over [t_start, t_end)
+// ---------------------------------------------------------------------------
+
+template <int LPC>
+__global__ __launch_bounds__(BLOCK) void k_emp_adaptive(Dev d, const int *list, int nlist, sedcol_params p,
+                                                        double t_start, double t_end, int r0, int r1,
+                                                        long long max_attempts, int *errflag)
+{
+  constexpr int G = BLOCK / LPC;
+  __shared__ double sbc[G][2 * NV];
+  __shared__ double sF[G][NV];
+  __shared__ double sFl[G][NV];
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  const bool valid = slot < nlist;
+  const int c = valid ? list[slot] : 0;
+  Lane L;
+  Geo Gm;
+  long long cell = 0;
+  double x[NV], fint[NV], acc[5];
+  sedcol_coldiag dg;
+  int64_t zs = 0;
+  bool dead = !valid;
+  double dt = 0.0;
+  if (valid) {
+    setup_lane<LPC>(d, c, l, L, Gm, cell);
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      x[n] = L.active ? d.conc[n * d.ncell + cell] : 0.0;
+      fint[n] = l == 0 ? d.colacc[(long long)n * d.ncol + c] : 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 5; ++a) acc[a] = L.active ? d.cellacc[a * d.ncell + cell] : 0.0;
+    dg = d.diag[c];
+    dt = dg.dt_next;
+    if (!(dt > 0.0)) dt = p.emp_dt_max;
+    if (l < NV) sFl[g][l] = d.flux[(long long)l * d.ncol + c];
+  }
+  const bool act = valid && L.active;
+  long long attempts = 0;
+
+  for (int r = r0; r <= r1; ++r) {
+    const double lo = (r == 0) ? -INFINITY : d.t0 + (double)r * d.dt_rec;
+    const double hi = (r == d.nrec - 1) ? INFINITY : d.t0 + (double)(r + 1) * d.dt_rec;
+    const double a = t_start > lo ? t_start : lo;
+    const double b = t_end < hi ? t_end : hi;
+    const double span = b - a;
+    __syncthreads();
+    if (valid) load_forcing<LPC>(d, p, c, r, Gm, L, sbc[g]);
+    __syncthreads();
+    if (dead || !(span > 0.0)) continue;
+    const double *bc = sbc[g];
+    double rem = span;
+    while (rem > 0.0) {
+      if (attempts >= max_attempts) {
+        dg.status = 2;
+        dead = true;
+        if (l == 0) atomicOr(errflag, 2);
+        break;
+      }
+      ++attempts;
+      const bool clipped = !(dt < rem);
+      const double h = clipped ? rem : dt;
+      double f0[NV], fc[NV], c1[NV];
+      double s0C, s0N, s1C, s1N;
+      column_rhs<LPC>(p, L, bc, x, s0C, s0N, [&](int n, double rr, double F) {
+        f0[n] = rr;
+        fc[n] = F;
+      });
+      const double p1 = psolve<LPC>(x, f0, h, act, zs);
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        const double y = x[n] + (h * p1) * f0[n];
+        c1[n] = y < 0.0 ? 0.0 : y;
+      }
+      column_rhs<LPC>(p, L, bc, c1, s1C, s1N, [&](int n, double rr, double F) {
+        f0[n] = (f0[n] + rr) * 0.5;
+        fc[n] = (fc[n] + F) * 0.5;
+        if (l == 0) sF[g][n] = F;
+      });
+      const double p2 = psolve<LPC>(x, f0, h, act, zs);
+      double el = 0.0;
+      bool bad = false;
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        const double y = x[n] + (h * p2) * f0[n];
+        const double yp = y < 0.0 ? 0.0 : y;
+        const double sc = p.emp_atol + p.emp_rtol * (yp > x[n] ? yp : x[n]);
+        const double e = fabs(yp - c1[n]) / sc;
+        if (act) {
+          bad |= e != e;
+          el = e > el ? e : el;
+        }
+      }
+      const double err = lg_max<LPC>(el);
+      if (group_any<LPC>(bad)) {
+        dg.status = 1;
+        dead = true;
+        if (l == 0) atomicOr(errflag, 1);
+        break;
+      }
+      const bool accept = (err <= 1.0) || !(h > p.emp_dt_min);
+      double fac = err > 0.0 ? 0.9 * drsqrt(err) : 5.0;
+      if (fac < 0.2) fac = 0.2;
+      if (fac > 5.0) fac = 5.0;
+      if (p1 < dg.p_min) dg.p_min = p1;
+      if (p2 < dg.p_min) dg.p_min = p2;
+      if (accept) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+          double y = x[n] + (h * p2) * f0[n];
+          if (y < 0.0) {
+            if (elem(n) >= 0) acc[2 + elem(n)] += 0.0 - y;
+            y = 0.0;
+          }
+          x[n] = y;
+          fint[n] += (h * p2) * fc[n];
+        }
+        acc[0] += (h * p2) * ((s0C + s1C) * 0.5);
+        acc[1] += (h * p2) * ((s0N + s1N) * 0.5);
+        if (l < NV) sFl[g][l] = sF[g][l];
+        rem = rem - h;
+        dg.accepted += 1;
+        if (err > 1.0) dg.forced += 1;
+        dg.dt_last = h;
+        if (h < dg.dt_minacc) dg.dt_minacc = h;
+        const double prop = h * fac;
+        dt = (clipped && dt > prop) ? dt : prop;
+      } else {
+        dg.rejected += 1;
+        dt = h * fac;
+      }
+      if (dt < p.emp_dt_min) dt = p.emp_dt_min;
+      if (dt > p.emp_dt_max) dt = p.emp_dt_max;
+    }
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    if (act) d.conc[n * d.ncell + cell] = x[n];
+    if (l == 0) d.colacc[(long long)n * d.ncol + c] = fint[n];
+  }
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+    if (act) d.cellacc[a * d.ncell + cell] = acc[a];
+  if (l < NV) d.flux[(long long)l * d.ncol + c] = sFl[g][l];
+  const double zsum = lg_sum<LPC>((double)zs);
+  if (l == 0) {
+    dg.zero_sink += (int64_t)zsum;
+    dg.dt_next = dt;
+    d.diag[c] = dg;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ADAPTIVE_EULER (solver_library.F90:104-140): the reject rule is global, so
+// each sub-step is a right-hand side, a domain-wide check and a commit
+// ---------------------------------------------------------------------------
+
+template <int LPC>
+__global__ __launch_bounds__(BLOCK) void k_rhs_store(Dev d, const int *list, int nlist, sedcol_params p, int rec,
+                                                     double *r_out, double *s_out)
+{
+  constexpr int G = BLOCK / LPC;
+  __shared__ double sbc[G][2 * NV];
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  const bool valid = slot < nlist;
+  const int c = valid ? list[slot] : 0;
+  Lane L;
+  Geo Gm;
+  long long cell = 0;
+  if (valid) {
+    setup_lane<LPC>(d, c, l, L, Gm, cell);
+    load_forcing<LPC>(d, p, c, rec, Gm, L, sbc[g]);
+  }
+  __syncthreads();
+  if (!valid) return;
+  double x[NV];
+#pragma unroll
+  for (int n = 0; n < NV; ++n) x[n] = L.active ? d.conc[n * d.ncell + cell] : 0.0;
+  double sC, sN;
+  column_rhs<LPC>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
+    if (L.active) r_out[n * d.ncell + cell] = r;
+    if (l == 0) d.flux[(long long)n * d.ncol + c] = F;
+  });
+  if (L.active) {
+    s_out[cell] = sC;
+    s_out[d.ncell + cell] = sN;
+  }
+}
+
+template <int LPC>
+__global__ __launch_bounds__(BLOCK) void k_ae_check(Dev d, const int *list, int nlist, double dt_red, double rcm,
+                                                    const double *r_in, int *viol)
+{
+  constexpr int G = BLOCK / LPC;
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  if (slot >= nlist) return;
+  const int c = list[slot];
+  const int o = d.off[c], K = d.off[c + 1] - o;
+  if (l >= K) return;
+  const long long cell = (long long)o + l;
+  bool v = false;
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    const double x = d.conc[n * d.ncell + cell];
+    const double c1 = x + dt_red * r_in[n * d.ncell + cell];
+    v |= (c1 - (1.0 + rcm) * x) < 0.0;
+  }
+  if (v) atomicOr(viol, 1);
+}
+
+template <int LPC>
+__global__ __launch_bounds__(BLOCK) void k_ae_commit(Dev d, const int *list, int nlist, double dt_red,
+                                                     const double *r_in, const double *s_in)
+{
+  constexpr int G = BLOCK / LPC;
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  if (slot >= nlist) return;
+  const int c = list[slot];
+  const int o = d.off[c], K = d.off[c + 1] - o;
+  if (l == 0) {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) d.colacc[(long long)n * d.ncol + c] += dt_red * d.flux[(long long)n * d.ncol + c];
+    sedcol_coldiag dg = d.diag[c];
+    dg.accepted += 1;
+    dg.dt_last = dt_red;
+    if (dt_red < dg.dt_minacc) dg.dt_minacc = dt_red;
+    d.diag[c] = dg;
+  }
+  if (l >= K) return;
+  const long long cell = (long long)o + l;
+#pragma unroll
+  for (int n = 0; n < NV; ++n) d.conc[n * d.ncell + cell] = d.conc[n * d.ncell + cell] + dt_red * r_in[n * d.ncell + cell];
+  d.cellacc[cell] += dt_red * s_in[cell];
+  d.cellacc[d.ncell + cell] += dt_red * s_in[d.ncell + cell];
+}
+
+// minimum clamp after an ode_solver call (fabm_sediment_component.F90:1726-1731)
+template <int LPC>
+__global__ __launch_bounds__(BLOCK) void k_clamp(Dev d, const int *list, int nlist, int *errflag)
+{
+  constexpr int G = BLOCK / LPC;
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  if (slot >= nlist) return;
+  const int c = list[slot];
+  const int o = d.off[c], K = d.off[c + 1] - o;
+  if (l >= K) return;
+  const long long cell = (long long)o + l;
+  bool nan = false;
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    const double x = d.conc[n * d.ncell + cell];
+    nan |= x != x;
+    if (x < 0.0) {
+      if (elem(n) >= 0) d.cellacc[(2 + elem(n)) * d.ncell + cell] += 0.0 - x;
+      d.conc[n * d.ncell + cell] = 0.0;
+    }
+  }
+  if (nan) atomicOr(errflag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// column reductions: budget, inventory, steady-state residual
+// ---------------------------------------------------------------------------
+
+template <int LPC>
+__global__ __launch_bounds__(BLOCK) void k_budget(Dev d, const int *list, int nlist, double *budget)
+{
+  constexpr int G = BLOCK / LPC;
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  if (slot >= nlist) return;
+  const int c = list[slot];
+  const int o = d.off[c], K = d.off[c + 1] - o;
+  const bool act = l < K;
+  const long long cell = (long long)o + l;
+  const double w = act ? d.phi[cell] * d.dz[cell] : 0.0;
+  double *b = budget + (long long)c * SEDCOL_NBUD;
+  if (l < NV) b[l] = d.colacc[(long long)l * d.ncol + c];
+#pragma unroll
+  for (int a = 0; a < 5; ++a) {
+    const double v = act ? w * d.cellacc[a * d.ncell + cell] : 0.0;
+    const double s = lg_sum<LPC>(v);
+    if (l == 0) b[NV + a] = s;
+  }
+}
+
+template <int LPC>
+__global__ __launch_bounds__(BLOCK) void k_inventory(Dev d, const int *list, int nlist, double *inv)
+{
+  constexpr int G = BLOCK / LPC;
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  if (slot >= nlist) return;
+  const int c = list[slot];
+  const int o = d.off[c], K = d.off[c + 1] - o;
+  const bool act = l < K;
+  const long long cell = (long long)o + l;
+  const double w = act ? d.phi[cell] * d.dz[cell] : 0.0;
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    const double v = act ? w * d.conc[n * d.ncell + cell] : 0.0;
+    const double s = lg_sum<LPC>(v);
+    if (l == 0) inv[(long long)n * d.ncol + c] = s;
+  }
+}
+
+template <int LPC>
+__global__ __launch_bounds__(BLOCK) void k_residual(Dev d, const int *list, int nlist, sedcol_params p, int rec,
+                                                    const double *cfloor, double *resid)
+{
+  constexpr int G = BLOCK / LPC;
+  __shared__ double sbc[G][2 * NV];
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  const bool valid = slot < nlist;
+  const int c = valid ? list[slot] : 0;
+  Lane L;
+  Geo Gm;
+  long long cell = 0;
+  if (valid) {
+    setup_lane<LPC>(d, c, l, L, Gm, cell);
+    load_forcing<LPC>(d, p, c, rec, Gm, L, sbc[g]);
+  }
+  __syncthreads();
+  if (!valid) return;
+  double x[NV];
+#pragma unroll
+  for (int n = 0; n < NV; ++n) x[n] = L.active ? d.conc[n * d.ncell + cell] : 0.0;
+  double sC, sN, m = 0.0;
+  column_rhs<LPC>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
+    const double ax = fabs(x[n]);
+    const double den = ax > cfloor[n] ? ax : cfloor[n];
+    const double e = fabs(r) / den;
+    if (L.active && (e > m || e != e)) m = e;
+  });
+  m = lg_max<LPC>(m);
+  if (l == 0) resid[c] = m;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers (host)
+// ---------------------------------------------------------------------------
+
+template <int LPC>
+static inline dim3 grid_for(int n)
+{
+  constexpr int G = BLOCK / LPC;
+  return dim3((unsigned)((n + G - 1) / G));
+}
+
+#define SC_DISPATCH_LPC(lpc, ...)                   \
+  switch (lpc) {                                    \
+    case 16: { constexpr int LPC = 16; __VA_ARGS__; } break; \
+    case 32: { constexpr int LPC = 32; __VA_ARGS__; } break; \
+    default: { constexpr int LPC = 64; __VA_ARGS__; } break; \
+  }
+
+void launch_rhs(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
+                const double *conc, double *rhs, double *flux)
+{
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_rhs<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n, p, rec,
+                                          conc, rhs, flux));
+}
+
+void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *list, int n,
+                  const sedcol_params &p, int rec, double dt, int nsteps, int *errflag)
+{
+#define SC_FIX(M) \
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_fixed<LPC, M>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n, p, \
+                                          rec, dt, nsteps, errflag))
+  switch (method) {
+    case SEDCOL_EULER: SC_FIX(SEDCOL_EULER); break;
+    case SEDCOL_RUNGE_KUTTA_4: SC_FIX(SEDCOL_RUNGE_KUTTA_4); break;
+    case SEDCOL_RUNGE_KUTTA_4_38: SC_FIX(SEDCOL_RUNGE_KUTTA_4_38); break;
+    case SEDCOL_EMP1: SC_FIX(SEDCOL_EMP1); break;
+    default: SC_FIX(SEDCOL_EMP2); break;
+  }
+#undef SC_FIX
+}
+
+void launch_emp_adaptive(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p,
+                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *errflag)
+{
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_emp_adaptive<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n,
+                                          p, t_start, t_end, r0, r1, max_attempts, errflag));
+}
+
+void launch_rhs_store(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
+                      double *r_out, double *s_out)
+{
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_rhs_store<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n, p,
+                                          rec, r_out, s_out));
+}
+
+void launch_ae_check(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double dt_red, double rcm,
+                     const double *r_in, int *viol)
+{
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_ae_check<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n,
+                                          dt_red, rcm, r_in, viol));
+}
+
+void launch_ae_commit(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double dt_red,
+                      const double *r_in, const double *s_in)
+{
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_ae_commit<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n,
+                                          dt_red, r_in, s_in));
+}
+
+void launch_clamp(int lpc, hipStream_t st, const Dev &d, const int *list, int n, int *errflag)
+{
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_clamp<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n,
+                                          errflag));
+}
+
+void launch_budget(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double *budget)
+{
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_budget<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n,
+                                          budget));
+}
+
+void launch_inventory(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double *inv)
+{
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_inventory<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n,
+                                          inv));
+}
+
+void launch_residual(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
+                     const double *cfloor, double *resid)
+{
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_residual<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n, p,
+                                          rec, cfloor, resid));
+}
+
+}  // namespace sc

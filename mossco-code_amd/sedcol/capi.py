@@ -1,0 +1,289 @@
+"""ctypes binding of libsedcol.so (include/sedcol.h) and the host-side
+mirror of the reference driver interface.
+
+``SedimentColumns`` plays the role of ``type_sed`` (src/drivers/
+fabm_sediment_driver.F90:69-113) together with ``ode_solver``
+(src/utilities/solver_library.F90:80-189): the same set-up calls, the same
+``get_rhs`` / ``ode_solver`` semantics, with the arrays living on the GPU.
+Errors raise ``SedcolError`` carrying the library's status code instead of
+the reference's ``stop``.
+
+The library is loaded from this package directory; there is no fallback:
+a missing or unloadable libsedcol.so raises at import of this module's
+``lib()``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .params import PARAM_FIELDS, NVAR, SedParams, fill_struct, params_dict
+
+LIB_NAME = "libsedcol.so"
+HERE = Path(__file__).resolve().parent
+
+SEDCOL_OK, SEDCOL_EINVAL, SEDCOL_EDOMAIN, SEDCOL_ENAN = 0, 1, 2, 3
+SEDCOL_EHIP, SEDCOL_ELIMIT, SEDCOL_ENOMEM, SEDCOL_EUNSUPPORTED = 4, 5, 6, 7
+LAYOUT_FORTRAN, LAYOUT_PACKED = 0, 1
+NBUD = 16
+
+#: every symbol include/sedcol.h declares
+EXPORTS = (
+    "sedcol_create", "sedcol_destroy", "sedcol_last_error", "sedcol_set_params", "sedcol_set_geometry",
+    "sedcol_set_forcing", "sedcol_set_forcing_series", "sedcol_set_state", "sedcol_get_state", "sedcol_rhs",
+    "sedcol_step", "sedcol_run", "sedcol_integrate", "sedcol_get_fluxes", "sedcol_get_budget",
+    "sedcol_reset_budget", "sedcol_get_inventory", "sedcol_get_diagnostics", "sedcol_reset_diagnostics",
+    "sedcol_residual", "sedcol_get_last_min_dt", "sedcol_synchronize", "sedcol_last_timing",
+)
+
+
+class SedcolError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sedcol status {code}: {msg}")
+        self.code = code
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("ncol", ctypes.c_int32), ("kmax", ctypes.c_int32), ("nvar", ctypes.c_int32),
+                ("layout", ctypes.c_int32), ("nlayers", ctypes.POINTER(ctypes.c_int32)),
+                ("mask", ctypes.POINTER(ctypes.c_int32))]
+
+
+class Params(ctypes.Structure):
+    _fields_ = PARAM_FIELDS
+
+
+class ColDiag(ctypes.Structure):
+    _fields_ = [("accepted", ctypes.c_int64), ("rejected", ctypes.c_int64), ("forced", ctypes.c_int64),
+                ("zero_sink", ctypes.c_int64), ("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double),
+                ("dt_minacc", ctypes.c_double), ("p_min", ctypes.c_double), ("status", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+DIAG_DTYPE = np.dtype([("accepted", np.int64), ("rejected", np.int64), ("forced", np.int64),
+                       ("zero_sink", np.int64), ("dt_last", np.float64), ("dt_next", np.float64),
+                       ("dt_minacc", np.float64), ("p_min", np.float64), ("status", np.int32),
+                       ("pad", np.int32)])
+
+_LIB = None
+
+
+def lib_path() -> Path:
+    return Path(os.environ.get("SEDCOL_LIB", HERE / LIB_NAME))
+
+
+def lib() -> ctypes.CDLL:
+    """Load libsedcol.so (raises OSError if it is missing)."""
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not path.exists():
+            raise OSError(f"{path} not found: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(str(path))
+        vp, i32, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+        pd = ctypes.POINTER(ctypes.c_double)
+        sig = {
+            "sedcol_create": (ctypes.c_int, [ctypes.POINTER(Desc), ctypes.POINTER(vp)]),
+            "sedcol_destroy": (ctypes.c_int, [vp]),
+            "sedcol_last_error": (ctypes.c_char_p, []),
+            "sedcol_set_params": (ctypes.c_int, [vp, ctypes.POINTER(Params)]),
+            "sedcol_set_geometry": (ctypes.c_int, [vp, pd, pd, pd, pd, pd, pd]),
+            "sedcol_set_forcing": (ctypes.c_int, [vp, pd, pd, pd]),
+            "sedcol_set_forcing_series": (ctypes.c_int, [vp, i32, dbl, dbl, pd, pd, pd]),
+            "sedcol_set_state": (ctypes.c_int, [vp, pd]),
+            "sedcol_get_state": (ctypes.c_int, [vp, pd]),
+            "sedcol_rhs": (ctypes.c_int, [vp, pd, pd, pd]),
+            "sedcol_step": (ctypes.c_int, [vp, dbl, i32, pd, pd]),
+            "sedcol_run": (ctypes.c_int, [vp, dbl, i32, i32, i32]),
+            "sedcol_integrate": (ctypes.c_int, [vp, dbl, dbl]),
+            "sedcol_get_fluxes": (ctypes.c_int, [vp, pd]),
+            "sedcol_get_budget": (ctypes.c_int, [vp, pd]),
+            "sedcol_reset_budget": (ctypes.c_int, [vp]),
+            "sedcol_get_inventory": (ctypes.c_int, [vp, pd]),
+            "sedcol_get_diagnostics": (ctypes.c_int, [vp, vp]),
+            "sedcol_reset_diagnostics": (ctypes.c_int, [vp]),
+            "sedcol_residual": (ctypes.c_int, [vp, pd, pd]),
+            "sedcol_get_last_min_dt": (ctypes.c_int, [vp, pd]),
+            "sedcol_synchronize": (ctypes.c_int, [vp]),
+            "sedcol_last_timing": (ctypes.c_int, [vp, pd, ctypes.POINTER(ctypes.c_int32)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def check(code: int):
+    if code != SEDCOL_OK:
+        msg = lib().sedcol_last_error()
+        raise SedcolError(code, msg.decode() if msg else "")
+
+
+class SedimentColumns:
+    """GPU sediment domain with the reference driver's call sequence.
+
+    layout "packed": arrays as in ``cases.Domain`` (state (NV, ncell));
+    layout "fortran": uniform K, state (NV, K, ncol) = Fortran conc(i,j,k,n)
+    with ncol = I*J, cell arrays (K, ncol).
+    """
+
+    def __init__(self, nlayers, mask=None, layout="packed"):
+        self.nlayers = _c(nlayers, np.int32)
+        self.ncol = int(self.nlayers.size)
+        self.kmax = int(self.nlayers.max())
+        self.layout = LAYOUT_PACKED if layout == "packed" else LAYOUT_FORTRAN
+        self.mask = _c(np.zeros(self.ncol) if mask is None else mask, np.int32)
+        self.ncell = int(self.nlayers.sum())
+        d = Desc()
+        d.ncol, d.kmax, d.nvar, d.layout = self.ncol, self.kmax, NVAR, self.layout
+        d.nlayers = self.nlayers.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if self.layout == LAYOUT_PACKED else None
+        d.mask = self.mask.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        h = ctypes.c_void_p()
+        check(lib().sedcol_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+
+    # -- lifecycle -----------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sedcol_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- set-up --------------------------------------------------------
+    def set_params(self, p: SedParams | dict):
+        s = fill_struct(Params, params_dict(p))
+        check(lib().sedcol_set_params(self._h, ctypes.byref(s)))
+
+    def set_geometry(self, dz, dzc, phi, phiI, bfac, biot):
+        arrs = [_c(a) for a in (dz, dzc, phi, phiI, bfac, biot)]
+        check(lib().sedcol_set_geometry(self._h, *[_pd(a) for a in arrs]))
+
+    def set_forcing(self, temp, cbw, fup):
+        t, c, f = _c(temp), _c(cbw), _c(fup)
+        check(lib().sedcol_set_forcing(self._h, _pd(t), _pd(c), _pd(f)))
+
+    def set_forcing_series(self, temp, cbw, fup, t0, dt_rec):
+        t, c, f = _c(temp), _c(cbw), _c(fup)
+        nrec = int(t.shape[0])
+        check(lib().sedcol_set_forcing_series(self._h, nrec, float(t0), float(dt_rec), _pd(t), _pd(c), _pd(f)))
+
+    def set_state(self, conc):
+        x = _c(conc)
+        check(lib().sedcol_set_state(self._h, _pd(x)))
+
+    def get_state(self) -> np.ndarray:
+        x = np.empty(self._state_shape())
+        check(lib().sedcol_get_state(self._h, _pd(x)))
+        return x
+
+    def _state_shape(self):
+        if self.layout == LAYOUT_PACKED:
+            return (NVAR, self.ncell)
+        return (NVAR, self.kmax, self.ncol)
+
+    # -- hot path ------------------------------------------------------
+    def get_rhs(self, conc):
+        """get_rhs: (rhs, top fluxes (NV, ncol))."""
+        x = _c(conc)
+        r = np.empty_like(x)
+        f = np.empty((NVAR, self.ncol))
+        check(lib().sedcol_rhs(self._h, _pd(x), _pd(r), _pd(f)))
+        return r, f
+
+    def ode_solver(self, conc, dt, method):
+        """One ode_solver call on a host state, updated in place; returns
+        the top fluxes of the last right-hand side."""
+        x = conc if (conc.flags.c_contiguous and conc.dtype == np.float64) else _c(conc)
+        f = np.empty((NVAR, self.ncol))
+        check(lib().sedcol_step(self._h, float(dt), int(method), _pd(x), _pd(f)))
+        if x is not conc:
+            conc[...] = x
+        return f
+
+    def run(self, dt, nsteps, method, steps_per_launch=0):
+        check(lib().sedcol_run(self._h, float(dt), int(nsteps), int(method), int(steps_per_launch)))
+
+    def integrate(self, t_start, t_end):
+        check(lib().sedcol_integrate(self._h, float(t_start), float(t_end)))
+
+    # -- outputs -------------------------------------------------------
+    def fluxes(self):
+        f = np.empty((NVAR, self.ncol))
+        check(lib().sedcol_get_fluxes(self._h, _pd(f)))
+        return f
+
+    def budget(self):
+        b = np.empty((self.ncol, NBUD))
+        check(lib().sedcol_get_budget(self._h, _pd(b)))
+        return b
+
+    def reset_budget(self):
+        check(lib().sedcol_reset_budget(self._h))
+
+    def inventory(self):
+        v = np.empty((NVAR, self.ncol))
+        check(lib().sedcol_get_inventory(self._h, _pd(v)))
+        return v
+
+    def diagnostics(self):
+        d = np.zeros(self.ncol, dtype=DIAG_DTYPE)
+        check(lib().sedcol_get_diagnostics(self._h, ctypes.c_void_p(d.ctypes.data)))
+        return d
+
+    def reset_diagnostics(self):
+        check(lib().sedcol_reset_diagnostics(self._h))
+
+    def residual(self, cfloor):
+        cf = _c(cfloor)
+        r = np.empty(self.ncol)
+        check(lib().sedcol_residual(self._h, _pd(cf), _pd(r)))
+        return r
+
+    def last_min_dt(self):
+        v = ctypes.c_double()
+        check(lib().sedcol_get_last_min_dt(self._h, ctypes.byref(v)))
+        return v.value
+
+    def last_timing(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int32()
+        check(lib().sedcol_last_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    # -- convenience ---------------------------------------------------
+    @classmethod
+    def from_domain(cls, dom, record=0):
+        """Create, configure and load a cases.Domain (packed layout)."""
+        s = cls(dom.nlayers, dom.mask, layout="packed")
+        s.set_params(dom.params)
+        s.set_geometry(dom.dz, dom.dzc, dom.phi, dom.phiI, dom.bfac, dom.biot)
+        if dom.nrec > 1:
+            s.set_forcing_series(dom.temp, dom.cbw, dom.fup, dom.t0, dom.dt_rec)
+        else:
+            s.set_forcing(*dom.record(record))
+        s.set_state(dom.conc0)
+        return s

@@ -1,0 +1,118 @@
+"""Sediment grid, porosity and bioturbation profiles (host set-up).
+
+Mirrors the set-up half of the reference driver, which runs once per
+domain on the host and whose results the library receives through
+``sedcol_set_geometry``:
+
+* ``init_grid``           src/drivers/fabm_sediment_driver.F90:127-177
+* porosity / bioturbation in ``initialize`` :278-303
+* ``update_porosity``     :393-442 (interface porosity :434-435)
+
+plus the geometric grid, exponential porosity and "constant then
+exponential" bioturbation profile that the benchmark configurations use.
+All arrays are per column, top layer first.  ``dzc[k]`` is the distance
+between the centres of layers k-1 and k (the reference's ``dzc(k-1)``);
+``dzc[0]`` is unused and set to 0.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def init_grid_linear(K: int, dzmin: float):
+    """Reference grid: layer heights grow linearly from dzmin, total depth
+    0.18 K/(K+1) m (fabm_sediment_driver.F90:147,159-168)."""
+    fac = 0.18 / ((K + 1) / 2.0 * dzmin) - 1.0
+    dz = np.empty(K)
+    for k in range(1, K + 1):
+        dz[k - 1] = (1.0 + (fac - 1.0) * (k - 1) / (K - 1)) * dzmin
+    return _centres(dz)
+
+
+def geometric_ratio(K: int, dz1: float, total: float) -> float:
+    """Ratio r with dz1 (r^K - 1)/(r - 1) = total (bisection to full
+    precision)."""
+    if abs(K * dz1 - total) <= 1e-15 * total:
+        return 1.0
+    def s(r):
+        return dz1 * K if r == 1.0 else dz1 * (r ** K - 1.0) / (r - 1.0)
+    lo, hi = (1.0, 2.0) if K * dz1 < total else (1e-6, 1.0)
+    while s(hi) < total:
+        hi *= 2.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if s(mid) < total:
+            lo = mid
+        else:
+            hi = mid
+        if hi - lo <= 1e-16 * hi:
+            break
+    return 0.5 * (lo + hi)
+
+
+def init_grid_geometric(K: int, dz1: float, total: float):
+    """Geometric layers: top layer dz1, constant growth ratio, given total."""
+    r = geometric_ratio(K, dz1, total)
+    dz = dz1 * r ** np.arange(K, dtype=np.float64)
+    return _centres(dz)
+
+
+def _centres(dz: np.ndarray):
+    K = dz.size
+    zi = np.empty(K + 1)
+    zc = np.empty(K)
+    zi[0] = 0.0
+    for k in range(K):
+        zc[k] = zi[k] + 0.5 * dz[k]
+        zi[k + 1] = zi[k] + dz[k]
+    dzc = np.zeros(K)
+    dzc[1:] = zc[1:] - zc[:-1]
+    return {"dz": dz, "zc": zc, "zi": zi, "dzc": dzc}
+
+
+def porosity_linear(zc, porosity_max=0.7, porosity_fac=0.9):
+    """fabm_sediment_driver.F90:280."""
+    return porosity_max * (1.0 - porosity_fac * np.asarray(zc))
+
+
+def porosity_exponential(zc, phi0, phi_inf, length=0.02):
+    """phi(z) = phi_inf + (phi0 - phi_inf) exp(-z / length)."""
+    return phi_inf + (phi0 - phi_inf) * np.exp(-np.asarray(zc) / length)
+
+
+def interface_porosity(phi):
+    """update_porosity :434-435: phiI_1 = phi_1, phiI_k = (phi_{k-1}+phi_k)/2."""
+    phi = np.asarray(phi, dtype=np.float64)
+    out = np.empty_like(phi)
+    out[0] = phi[0]
+    out[1:] = 0.5 * (phi[:-1] + phi[1:])
+    return out
+
+
+def bioturbation_factor(zi_upper, profile, bioturbation=0.9, depth_cm=5.0, min_cm2d=0.2):
+    """Bioturbation factor at the upper interface of each layer
+    (fabm_sediment_driver.F90:293-303): 0 constant, 1 linear decrease to
+    bioturbation_min, 2 exponential."""
+    zi = np.asarray(zi_upper, dtype=np.float64)
+    if profile == 0:
+        return np.ones_like(zi)
+    if profile == 1:
+        return np.maximum(min_cm2d / bioturbation, np.maximum(depth_cm - 100.0 * zi, 0.0) / depth_cm)
+    if profile == 2:
+        return np.exp(-100.0 * zi / depth_cm)
+    raise ValueError("bioturbation profile 3 (Zhang & Wirtz) is not supported")
+
+
+def bioturbation_const_exp(zi_upper, mixed_depth=0.05, length=0.05):
+    """1 down to mixed_depth, then exp(-(z - mixed_depth)/length)."""
+    zi = np.asarray(zi_upper, dtype=np.float64)
+    return np.where(zi <= mixed_depth, 1.0, np.exp(-(zi - mixed_depth) / length))
+
+
+def flux_cap(phi, dz, pom_flux_max=2.0e4):
+    """Distributed-POM flux cap (fabm_sediment_driver.F90:284-291)."""
+    fc = pom_flux_max / 86400.0 * (1.0 - np.asarray(phi)) * np.asarray(dz)
+    for k in range(2, fc.size):
+        if fc[k] > fc[k - 1]:
+            fc[k] = fc[k - 1]
+    return fc

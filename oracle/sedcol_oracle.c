@@ -1,0 +1,871 @@
+/*
+ * sedcol_oracle.c -- CPU oracle for the sediment-column hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see sedcol_oracle.h).  Only tests/, the smoke
+ * check and the cpu_baseline leg of bench.py load this library, and only as
+ * the checker.
+ *
+ * Floating point: compiled with -ffp-contract=off; every fused multiply-add
+ * is an explicit fma() call, so the operation order written here is the one
+ * executed.  Reductions over the layers of a column use a balanced pairwise
+ * tree over the next power of two of K (identity padding), the order a
+ * lane-per-layer butterfly produces.
+ */
+#include "sedcol_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NV OC_NVAR
+#define OC_KMAX 128
+
+/* particulate flag per state variable: detritus pools and adsorbed P are
+ * attached to the solid phase (fabm_sediment_driver.F90:658 'particulate') */
+static const int oc_part[NV] = {1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 1};
+/* element membership for clamp accounting: 0 = C, 1 = N, 2 = P, -1 none */
+static const int oc_elem[NV] = {0, 0, 1, 1, 2, -1, 1, 1, -1, 2, 2};
+
+/* ------------------------------------------------------------------------ */
+/* elementary functions with a fixed operation sequence                      */
+/* ------------------------------------------------------------------------ */
+
+/* exp(x): Cody-Waite reduction x = n ln2 + r with the fdlibm two-part ln2,
+ * then the Taylor series of e^r to degree 13 in Horner form (|r| <= 0.347,
+ * truncation < 3e-18 relative), scaled by 2^n through the exponent field.
+ * Normal-range results only (x in [-708, 709]). */
+double oc_exp(double x)
+{
+  if (x != x) return x;
+  if (x > 709.0) return INFINITY;
+  if (x < -708.0) return 0.0;
+  const double log2e = 1.4426950408889634;
+  const double ln2hi = 6.93147180369123816490e-01;
+  const double ln2lo = 1.90821492927058770002e-10;
+  double n = rint(x * log2e);
+  double r = (x - n * ln2hi) - n * ln2lo;
+  double s = 1.6059043836821613e-10;          /* 1/13! */
+  s = fma(s, r, 2.08767569878681e-09);        /* 1/12! */
+  s = fma(s, r, 2.505210838544172e-08);       /* 1/11! */
+  s = fma(s, r, 2.755731922398589e-07);       /* 1/10! */
+  s = fma(s, r, 2.7557319223985893e-06);      /* 1/9!  */
+  s = fma(s, r, 2.48015873015873e-05);        /* 1/8!  */
+  s = fma(s, r, 0.0001984126984126984);       /* 1/7!  */
+  s = fma(s, r, 0.001388888888888889);        /* 1/6!  */
+  s = fma(s, r, 0.008333333333333333);        /* 1/5!  */
+  s = fma(s, r, 0.041666666666666664);        /* 1/4!  */
+  s = fma(s, r, 0.16666666666666666);         /* 1/3!  */
+  s = fma(s, r, 0.5);
+  s = fma(s, r, 1.0);
+  s = fma(s, r, 1.0);
+  int64_t e = (int64_t)n + 1023;
+  uint64_t bits = (uint64_t)e << 52;
+  double two_n;
+  memcpy(&two_n, &bits, sizeof two_n);
+  return s * two_n;
+}
+
+/* 1/sqrt(x) for finite x > 0: exponent-halving initial guess followed by four
+ * Newton steps y <- y (1.5 - (x/2) y^2).  Used only by the step-size
+ * controller, where determinism matters more than the last bit. */
+double oc_rsqrt(double x)
+{
+  uint64_t bits;
+  memcpy(&bits, &x, sizeof bits);
+  bits = 0x5FE6EB50C7B537A9ull - (bits >> 1);
+  double y;
+  memcpy(&y, &bits, sizeof y);
+  double h = 0.5 * x;
+  for (int i = 0; i < 4; ++i) y = y * (1.5 - (h * y) * y);
+  return y;
+}
+
+static inline double nn(double c) { return c < 0.0 ? 0.0 : c; }
+
+/* temperature factor, fabm_sediment_driver.F90:648 (the _ONE_* factor is an
+ * exact multiplication by one) */
+static inline double oc_fT(double T)
+{
+  return oc_exp(-4500.0 * (1.0 / (T + 273.0) - (1.0 / 288.0)));
+}
+
+/* ------------------------------------------------------------------------ */
+/* kinetics (project definition; FABM hzg_omexdia_p is not in the reference) */
+/* ------------------------------------------------------------------------ */
+
+/* Local rates R[n] (mmol m-3 pore water s-1) at one cell, following the
+ * OMEXDIA scheme of Soetaert et al. (1996): first-order detritus decay with a
+ * saturating production cap (CprodMax), partition of carbon production into
+ * oxic mineralisation, denitrification and anoxic mineralisation by
+ * Monod/inhibition terms normalised to one, nitrification and ODU
+ * re-oxidation, and P adsorption onto / release from oxidised iron.
+ * Parameter names follow examples/standalone/omexdia_p/fabm_sed.nml:51-77.
+ * Every destruction term of a species is proportional to that species, which
+ * is what the Patankar steppers require.  sinkC = carbon leaving as DIC,
+ * sinkN = nitrogen leaving as N2. */
+void oc_kinetics(const oc_params *p, double fT, const double *c, double *R,
+                 double *sinkC, double *sinkN)
+{
+  double ldC = nn(c[OC_LDETC]), sdC = nn(c[OC_SDETC]);
+  double ldN = nn(c[OC_LDETN]), sdN = nn(c[OC_SDETN]);
+  double dP = nn(c[OC_DETP]), O2 = nn(c[OC_O2]), NO3 = nn(c[OC_NO3]);
+  double NH4 = nn(c[OC_NH4]), ODU = nn(c[OC_ODU]), PO4 = nn(c[OC_PO4]);
+  double aP = nn(c[OC_ADSP]);
+
+  double kL = fT * p->rL;
+  double kS = fT * p->rS;
+  double dL = kL * ldC;
+  double dS = kS * sdC;
+  double lim = p->CprodMax / (p->CprodMax + (dL + dS));
+  double CpL = dL * lim;
+  double CpS = dS * lim;
+  double Cprod = CpL + CpS;
+  double NpL = (kL * ldN) * lim;
+  double NpS = (kS * sdN) * lim;
+  double Pprod = (kL * dP) * lim;
+
+  double oxlim = O2 / (O2 + p->ksO2oxic);
+  double dnlim = (p->kinO2denit / (O2 + p->kinO2denit)) * (NO3 / (NO3 + p->ksNO3denit));
+  double anlim = (p->kinO2anox / (O2 + p->kinO2anox)) * (p->kinNO3anox / (NO3 + p->kinNO3anox));
+  double rescale = 1.0 / ((oxlim + dnlim) + anlim);
+  double OxicMin = (Cprod * oxlim) * rescale;
+  double Denit = (Cprod * dnlim) * rescale;
+  double AnoxMin = (Cprod * anlim) * rescale;
+
+  double Nitri = ((fT * p->rnit) * NH4) * (O2 / (O2 + p->ksO2nitri));
+  double o2odu = O2 / (O2 + p->ksO2oduox);
+  double OduOx = ((fT * p->rODUox) * ODU) * o2odu;
+  double kP = fT * p->rPads;
+  double Pads = (kP * PO4) * o2odu;
+  double Pdes = (kP * aP) * (ODU / (ODU + p->PAdsODU));
+
+  R[OC_LDETC] = -CpL;
+  R[OC_SDETC] = -CpS;
+  R[OC_LDETN] = -NpL;
+  R[OC_SDETN] = -NpS;
+  R[OC_DETP] = -Pprod;
+  R[OC_O2] = -((OxicMin + 2.0 * Nitri) + OduOx);
+  R[OC_NO3] = Nitri - 0.8 * Denit;
+  R[OC_NH4] = (NpL + NpS) - Nitri;
+  R[OC_ODU] = AnoxMin - OduOx;
+  R[OC_PO4] = (Pprod - Pads) + Pdes;
+  R[OC_ADSP] = Pads - Pdes;
+  *sinkC = Cprod;
+  *sinkN = 0.8 * Denit;
+}
+
+/* ------------------------------------------------------------------------ */
+/* column view and column right-hand side                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int K;
+  const double *dz, *dzc, *phi, *phiI, *bfac, *fcap;
+  double B, T;
+  double cbw[NV], fup[NV];
+} colview;
+
+static void make_view(const oc_domain *d, int c, colview *v)
+{
+  int o = d->off[c];
+  v->K = d->off[c + 1] - o;
+  v->dz = d->dz + o;
+  v->dzc = d->dzc + o;
+  v->phi = d->phi + o;
+  v->phiI = d->phiI + o;
+  v->bfac = d->bfac + o;
+  v->fcap = d->fcap ? d->fcap + o : NULL;
+  v->B = d->biot[c];
+  v->T = d->temp[c];
+  for (int n = 0; n < NV; ++n) {
+    v->cbw[n] = d->cbw[(size_t)n * d->ncol + c];
+    v->fup[n] = d->fluxup[(size_t)n * d->ncol + c];
+  }
+}
+
+/* One get_rhs for one column.  c and rhs are [NV][K] with stride ks.
+ * F1[n] = top interface flux (intFlux(:,:,1)); sC/sN per-cell sinks.
+ * fabm_sediment_driver.F90:575-717 and diff3d :739-825. */
+static void col_rhs(const oc_params *p, const colview *v, const double *c,
+                    long ks, double *rhs, long rs, double *F1, double *sC,
+                    double *sN)
+{
+  const int K = v->K;
+  double Dbio[OC_KMAX], Ddis[OC_KMAX], x[OC_KMAX], F[OC_KMAX + 1];
+  double fT = oc_fT(v->T);
+  /* :652-653, then :682-683 for dissolved variables */
+  for (int k = 0; k < K; ++k) {
+    Dbio[k] = ((((v->B * fT) / 86400.0) / 10000.0) * (1.0 - v->phiI[k])) * v->bfac[k];
+    Ddis[k] = Dbio[k] + (((p->D0 + v->T * 0.035) * v->phiI[k]) / 86400.0) / 10000.0;
+  }
+  for (int n = 0; n < NV; ++n) {
+    const int part = oc_part[n];
+    const double *D = part ? Dbio : Ddis;
+    const double *cn = c + (long)n * ks;
+    for (int k = 0; k < K; ++k) x[k] = part ? cn[k] * v->phi[k] : cn[k];
+    /* interior interfaces :776-778 */
+    for (int k = 1; k < K; ++k) F[k] = -((D[k] * (x[k] - x[k - 1])) / v->dzc[k]);
+    /* upper boundary :782-803 */
+    int bc = part ? p->bcup_part : p->bcup_dis;
+    if (bc == 1) {
+      F[0] = v->fup[n];
+    } else if (bc == 2) {
+      F[0] = -((D[0] * (x[0] - v->cbw[n])) / v->dz[0]);
+    } else if (bc == 3) {
+      F[0] = 0.0;
+    } else { /* 4: capped flux, overflow cascades downward */
+      F[0] = v->fup[n];
+      int k = 1;
+      double rest = F[0] - v->fcap[0];
+      while (rest > 0.0 && k < K) {
+        F[k] = F[k] + rest;
+        rest = rest - v->fcap[k];
+        k = k + 1;
+      }
+      if (k >= K) F[K - 1] = F[K - 1] + rest;
+    }
+    /* lower boundary :806-815 (fluxdown = Cdown = 0 in get_rhs) */
+    if (p->bcdown == 1) F[K] = 0.0;
+    else if (p->bcdown == 2) F[K] = -((D[K - 1] * (0.0 - x[K - 1])) / v->dz[K - 1]);
+    else F[K] = 0.0;
+    /* divergence :818-820 and particulate rescale :677-678 */
+    double *rn = rhs + (long)n * rs;
+    for (int k = 0; k < K; ++k) {
+      double VF = part ? (1.0 - v->phi[k]) : v->phi[k];
+      double dC = (F[k] - F[k + 1]) / (VF * v->dz[k]);
+      rn[k] = part ? (dC * (1.0 - v->phi[k])) / v->phi[k] : dC;
+    }
+    F1[n] = F[0];
+  }
+  /* local rates, then rhs = R + transport (:697-715) */
+  for (int k = 0; k < K; ++k) {
+    double cc[NV], R[NV], a, b;
+    for (int n = 0; n < NV; ++n) cc[n] = c[(long)n * ks + k];
+    oc_kinetics(p, fT, cc, R, &a, &b);
+    for (int n = 0; n < NV; ++n) rhs[(long)n * rs + k] = R[n] + rhs[(long)n * rs + k];
+    sC[k] = a;
+    sN[k] = b;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* layer reductions (balanced pairwise tree, identity padding)              */
+/* ------------------------------------------------------------------------ */
+
+static int pow2ceil(int K)
+{
+  int q = 1;
+  while (q < K) q <<= 1;
+  return q;
+}
+
+static double tree_sum(double *v, int K)
+{
+  int q = pow2ceil(K);
+  for (int k = K; k < q; ++k) v[k] = 0.0;
+  for (int s = 1; s < q; s <<= 1)
+    for (int l = 0; l < q; l += 2 * s) v[l] = v[l] + v[l + s];
+  return v[0];
+}
+
+static void tree_pd(double *P, double *D, int K)
+{
+  int q = pow2ceil(K);
+  for (int k = K; k < q; ++k) { P[k] = 1.0; D[k] = 0.0; }
+  for (int s = 1; s < q; s <<= 1)
+    for (int l = 0; l < q; l += 2 * s) {
+      double p1 = P[l], d1 = D[l], p2 = P[l + s], d2 = D[l + s];
+      P[l] = p1 * p2;
+      D[l] = d1 * p2 + p1 * d2;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Patankar factor                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Root p in (0,1] of p = prod_{j: g_j<0} (1 + dt g_j p / c_j) over all
+ * components of one column (Bruggeman et al. 2007, eq. for EMP).  h(p) =
+ * prod - p is convex and decreasing on the admissible interval, so Newton
+ * from p = 0 increases monotonically to the root; the product and its
+ * derivative are carried together as (P, D) pairs. */
+static double col_psolve(const double *c, long ks, const double *g, long gs,
+                         double dt, int K, int64_t *zero_sink)
+{
+  double a[NV][OC_KMAX];
+  for (int n = 0; n < NV; ++n)
+    for (int k = 0; k < K; ++k) {
+      double gv = g[(long)n * gs + k], cv = c[(long)n * ks + k];
+      double av = 0.0;
+      if (gv < 0.0) {
+        if (cv > 0.0) av = (dt * gv) / cv;
+        else ++*zero_sink;
+      }
+      a[n][k] = av;
+    }
+  double pp = 0.0;
+  double P[OC_KMAX], D[OC_KMAX];
+  for (int it = 0; it < 60; ++it) {
+    for (int k = 0; k < K; ++k) {
+      double Pl = 1.0, Dl = 0.0;
+      for (int n = 0; n < NV; ++n) {
+        double av = a[n][k];
+        double f = fma(av, pp, 1.0);
+        Dl = Dl * f + Pl * av;
+        Pl = Pl * f;
+      }
+      P[k] = Pl;
+      D[k] = Dl;
+    }
+    tree_pd(P, D, K);
+    double h = P[0] - pp;
+    double hp = D[0] - 1.0;
+    double pn = pp - h / hp;
+    if (!(pn > pp)) break;
+    int conv = (pn - pp) <= 0x1p-50 * pn;
+    pp = pn;
+    if (conv) break;
+  }
+  if (pp > 1.0) pp = 1.0;
+  return pp;
+}
+
+/* ------------------------------------------------------------------------ */
+/* accumulators                                                              */
+/* cellacc[a*ncell + cell]: a = 0 sinkC, 1 sinkN, 2 clampC, 3 clampN, 4 clampP */
+/* colacc[n*ncol + c]: time-integrated top flux per variable                  */
+/* ------------------------------------------------------------------------ */
+
+static void clamp_col(const colview *v, double *c, long ks, double *cellacc,
+                      long ncell)
+{
+  for (int n = 0; n < NV; ++n)
+    for (int k = 0; k < v->K; ++k) {
+      double *x = c + (long)n * ks + k;
+      if (*x < 0.0) {
+        double add = 0.0 - *x;
+        if (cellacc && oc_elem[n] >= 0) cellacc[(2 + oc_elem[n]) * ncell + k] += add;
+        *x = 0.0;
+      }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* fixed-step methods for one column (0, 1, 3, 4, 5)                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  double r[4][NV * OC_KMAX];
+  double s[4][2][OC_KMAX];
+  double F[4][NV];
+  double c1[NV * OC_KMAX];
+} colwork;
+
+/* One ode_solver step of one column.  c: [NV][K] stride ks; cellacc points
+ * at this column's first cell (stride ncell); colacc at this column (stride
+ * ncol).  Returns 0, or 1 if the new state has a NaN. */
+static int col_step(const oc_params *p, const colview *v, int method,
+                    double dt, double *c, long ks, double *F1out,
+                    double *cellacc, long ncell, double *colacc, long ncol,
+                    oc_coldiag *dg, colwork *w)
+{
+  const int K = v->K;
+  const long S = OC_KMAX; /* stride of the work arrays */
+  const double third = 1.0 / 3.0;
+  double *c1 = w->c1;
+  double sc = 0.0, sn = 0.0; /* unused placeholders */
+  (void)sc; (void)sn;
+  if (method == OC_EULER) {
+    col_rhs(p, v, c, ks, w->r[0], S, w->F[0], w->s[0][0], w->s[0][1]);
+    for (int n = 0; n < NV; ++n)
+      for (int k = 0; k < K; ++k) c[n * ks + k] = c[n * ks + k] + dt * w->r[0][n * S + k];
+    if (colacc) for (int n = 0; n < NV; ++n) colacc[n * ncol] += dt * w->F[0][n];
+    if (cellacc)
+      for (int k = 0; k < K; ++k) {
+        cellacc[k] += dt * w->s[0][0][k];
+        cellacc[ncell + k] += dt * w->s[0][1][k];
+      }
+    memcpy(F1out, w->F[0], sizeof(double) * NV);
+  } else if (method == OC_RK4 || method == OC_RK4_38) {
+    const int rk38 = method == OC_RK4_38;
+    /* solver_library.F90:142-163 (RK4), :164-185 (3/8 rule) */
+    col_rhs(p, v, c, ks, w->r[0], S, w->F[0], w->s[0][0], w->s[0][1]);
+    for (int n = 0; n < NV; ++n)
+      for (int k = 0; k < K; ++k) {
+        double k1 = w->r[0][n * S + k];
+        c1[n * S + k] = rk38 ? c[n * ks + k] + (third * dt) * k1
+                             : c[n * ks + k] + (0.5 * dt) * k1;
+      }
+    col_rhs(p, v, c1, S, w->r[1], S, w->F[1], w->s[1][0], w->s[1][1]);
+    for (int n = 0; n < NV; ++n)
+      for (int k = 0; k < K; ++k) {
+        double k1 = w->r[0][n * S + k], k2 = w->r[1][n * S + k];
+        c1[n * S + k] = rk38 ? c[n * ks + k] + dt * (k2 - third * k1)
+                             : c[n * ks + k] + (0.5 * dt) * k2;
+      }
+    col_rhs(p, v, c1, S, w->r[2], S, w->F[2], w->s[2][0], w->s[2][1]);
+    for (int n = 0; n < NV; ++n)
+      for (int k = 0; k < K; ++k) {
+        double k1 = w->r[0][n * S + k], k2 = w->r[1][n * S + k], k3 = w->r[2][n * S + k];
+        c1[n * S + k] = rk38 ? c[n * ks + k] + dt * ((k1 - k2) + k3)
+                             : c[n * ks + k] + dt * k3;
+      }
+    col_rhs(p, v, c1, S, w->r[3], S, w->F[3], w->s[3][0], w->s[3][1]);
+    const double wdt = rk38 ? (dt * 1.0) / 8.0 : dt * third;
+#define COMB(a0, a1, a2, a3) \
+  (rk38 ? (((a0) + 3.0 * (a1)) + 3.0 * (a2)) + (a3) : (((0.5 * (a0) + (a1)) + (a2)) + 0.5 * (a3)))
+    for (int n = 0; n < NV; ++n)
+      for (int k = 0; k < K; ++k) {
+        long i = n * S + k;
+        c[n * ks + k] = c[n * ks + k] + wdt * COMB(w->r[0][i], w->r[1][i], w->r[2][i], w->r[3][i]);
+      }
+    if (colacc)
+      for (int n = 0; n < NV; ++n)
+        colacc[n * ncol] += wdt * COMB(w->F[0][n], w->F[1][n], w->F[2][n], w->F[3][n]);
+    if (cellacc)
+      for (int a = 0; a < 2; ++a)
+        for (int k = 0; k < K; ++k)
+          cellacc[a * ncell + k] += wdt * COMB(w->s[0][a][k], w->s[1][a][k], w->s[2][a][k], w->s[3][a][k]);
+#undef COMB
+    memcpy(F1out, w->F[3], sizeof(double) * NV);
+  } else { /* EMP-1 / EMP-2, fixed step */
+    col_rhs(p, v, c, ks, w->r[0], S, w->F[0], w->s[0][0], w->s[0][1]);
+    double p1 = col_psolve(c, ks, w->r[0], S, dt, K, &dg->zero_sink);
+    if (p1 < dg->p_min) dg->p_min = p1;
+    if (method == OC_EMP1) {
+      for (int n = 0; n < NV; ++n)
+        for (int k = 0; k < K; ++k) c[n * ks + k] = c[n * ks + k] + (dt * p1) * w->r[0][n * S + k];
+      if (colacc) for (int n = 0; n < NV; ++n) colacc[n * ncol] += (dt * p1) * w->F[0][n];
+      if (cellacc)
+        for (int k = 0; k < K; ++k) {
+          cellacc[k] += (dt * p1) * w->s[0][0][k];
+          cellacc[ncell + k] += (dt * p1) * w->s[0][1][k];
+        }
+      memcpy(F1out, w->F[0], sizeof(double) * NV);
+    } else {
+      for (int n = 0; n < NV; ++n)
+        for (int k = 0; k < K; ++k) {
+          double y = c[n * ks + k] + (dt * p1) * w->r[0][n * S + k];
+          c1[n * S + k] = y < 0.0 ? 0.0 : y;
+        }
+      col_rhs(p, v, c1, S, w->r[1], S, w->F[1], w->s[1][0], w->s[1][1]);
+      for (int n = 0; n < NV; ++n)
+        for (int k = 0; k < K; ++k)
+          w->r[2][n * S + k] = (w->r[0][n * S + k] + w->r[1][n * S + k]) * 0.5;
+      double p2 = col_psolve(c, ks, w->r[2], S, dt, K, &dg->zero_sink);
+      if (p2 < dg->p_min) dg->p_min = p2;
+      for (int n = 0; n < NV; ++n)
+        for (int k = 0; k < K; ++k) c[n * ks + k] = c[n * ks + k] + (dt * p2) * w->r[2][n * S + k];
+      if (colacc)
+        for (int n = 0; n < NV; ++n) colacc[n * ncol] += (dt * p2) * ((w->F[0][n] + w->F[1][n]) * 0.5);
+      if (cellacc)
+        for (int a = 0; a < 2; ++a)
+          for (int k = 0; k < K; ++k)
+            cellacc[a * ncell + k] += (dt * p2) * ((w->s[0][a][k] + w->s[1][a][k]) * 0.5);
+      memcpy(F1out, w->F[1], sizeof(double) * NV);
+    }
+    /* Patankar keeps components non-negative up to rounding */
+    clamp_col(v, c, ks, cellacc, ncell);
+  }
+  dg->accepted += 1;
+  dg->dt_last = dt;
+  if (dt < dg->dt_minacc) dg->dt_minacc = dt;
+  for (int n = 0; n < NV; ++n)
+    for (int k = 0; k < K; ++k)
+      if (c[n * ks + k] != c[n * ks + k]) return 1;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* adaptive EMP for one column                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Integrates one column over t_span with the embedded EMP-1/EMP-2 pair.
+ * err = max_j |c2_j - c1_j| / (atol + rtol max(c2_j, c_j)); accept when
+ * err <= 1 or the step is already at dt_min; next step dt * clamp(0.9
+ * err^-1/2, 0.2, 5) within [dt_min, dt_max]; a step clipped to the end of the
+ * span keeps the larger of the old and the proposed size. */
+static int col_emp_adaptive(const oc_params *p, const colview *v, double *c,
+                            long ks, double t_span, double *F1out,
+                            double *cellacc, long ncell, double *colacc,
+                            long ncol, oc_coldiag *dg, int64_t max_attempts,
+                            colwork *w)
+{
+  const int K = v->K;
+  const long S = OC_KMAX;
+  double *c1 = w->c1;
+  double *g = w->r[2];
+  double *cn = w->r[3];
+  double rem = t_span;
+  double dt = dg->dt_next;
+  if (!(dt > 0.0)) dt = p->emp_dt_max;
+  int64_t attempts = 0;
+  while (rem > 0.0) {
+    if (attempts >= max_attempts) { dg->status = 2; break; }
+    ++attempts;
+    int clipped = !(dt < rem);
+    double h = clipped ? rem : dt;
+    col_rhs(p, v, c, ks, w->r[0], S, w->F[0], w->s[0][0], w->s[0][1]);
+    double p1 = col_psolve(c, ks, w->r[0], S, h, K, &dg->zero_sink);
+    for (int n = 0; n < NV; ++n)
+      for (int k = 0; k < K; ++k) {
+        double y = c[n * ks + k] + (h * p1) * w->r[0][n * S + k];
+        c1[n * S + k] = y < 0.0 ? 0.0 : y;
+      }
+    col_rhs(p, v, c1, S, w->r[1], S, w->F[1], w->s[1][0], w->s[1][1]);
+    for (int n = 0; n < NV; ++n)
+      for (int k = 0; k < K; ++k)
+        g[n * S + k] = (w->r[0][n * S + k] + w->r[1][n * S + k]) * 0.5;
+    double p2 = col_psolve(c, ks, g, S, h, K, &dg->zero_sink);
+    double err = 0.0;
+    int bad = 0;
+    for (int n = 0; n < NV; ++n)
+      for (int k = 0; k < K; ++k) {
+        double y = c[n * ks + k] + (h * p2) * g[n * S + k];
+        cn[n * S + k] = y;
+        double yp = y < 0.0 ? 0.0 : y;
+        double c0 = c[n * ks + k];
+        double sc = p->emp_atol + p->emp_rtol * (yp > c0 ? yp : c0);
+        double e = fabs(yp - c1[n * S + k]) / sc;
+        if (e != e) bad = 1;
+        if (e > err) err = e;
+      }
+    if (bad) { dg->status = 1; break; }
+    int accept = (err <= 1.0) || !(h > p->emp_dt_min);
+    double fac = err > 0.0 ? 0.9 * oc_rsqrt(err) : 5.0;
+    if (fac < 0.2) fac = 0.2;
+    if (fac > 5.0) fac = 5.0;
+    if (p1 < dg->p_min) dg->p_min = p1;
+    if (p2 < dg->p_min) dg->p_min = p2;
+    if (accept) {
+      for (int n = 0; n < NV; ++n) {
+        for (int k = 0; k < K; ++k) {
+          double y = cn[n * S + k];
+          if (y < 0.0) {
+            if (cellacc && oc_elem[n] >= 0) cellacc[(2 + oc_elem[n]) * ncell + k] += 0.0 - y;
+            y = 0.0;
+          }
+          c[n * ks + k] = y;
+        }
+      }
+      if (colacc)
+        for (int n = 0; n < NV; ++n) colacc[n * ncol] += (h * p2) * ((w->F[0][n] + w->F[1][n]) * 0.5);
+      if (cellacc)
+        for (int a = 0; a < 2; ++a)
+          for (int k = 0; k < K; ++k)
+            cellacc[a * ncell + k] += (h * p2) * ((w->s[0][a][k] + w->s[1][a][k]) * 0.5);
+      memcpy(F1out, w->F[1], sizeof(double) * NV);
+      rem = rem - h;
+      dg->accepted += 1;
+      if (err > 1.0) dg->forced += 1;
+      dg->dt_last = h;
+      if (h < dg->dt_minacc) dg->dt_minacc = h;
+      double prop = h * fac;
+      dt = (clipped && dt > prop) ? dt : prop;
+    } else {
+      dg->rejected += 1;
+      dt = h * fac;
+    }
+    if (dt < p->emp_dt_min) dt = p->emp_dt_min;
+    if (dt > p->emp_dt_max) dt = p->emp_dt_max;
+  }
+  dg->dt_next = dt;
+  return dg->status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* domain-level entry points                                                 */
+/* ------------------------------------------------------------------------ */
+
+static int check_domain(const oc_domain *d)
+{
+  if (d->ncol < 0) return -1;
+  for (int c = 0; c < d->ncol; ++c) {
+    int K = d->off[c + 1] - d->off[c];
+    if (K < 1 || K > OC_KMAX) return -2;
+  }
+  return 0;
+}
+
+/* get_rhs for the whole domain; masked columns get rhs = 0 and flux = 0 */
+int oc_rhs(const oc_params *p, const oc_domain *d, const double *conc,
+           double *rhs, double *flux_out)
+{
+  if (check_domain(d)) return -1;
+  const long ncell = d->ncell;
+  for (int c = 0; c < d->ncol; ++c) {
+    int o = d->off[c];
+    int K = d->off[c + 1] - o;
+    double F1[NV], sC[OC_KMAX], sN[OC_KMAX];
+    if (d->mask[c]) {
+      for (int n = 0; n < NV; ++n) {
+        for (int k = 0; k < K; ++k) rhs[n * ncell + o + k] = 0.0;
+        if (flux_out) flux_out[(long)n * d->ncol + c] = 0.0;
+      }
+      continue;
+    }
+    colview v;
+    make_view(d, c, &v);
+    col_rhs(p, &v, conc + o, ncell, rhs + o, ncell, F1, sC, sN);
+    if (flux_out)
+      for (int n = 0; n < NV; ++n) flux_out[(long)n * d->ncol + c] = F1[n];
+  }
+  return 0;
+}
+
+static void diag_init(oc_coldiag *dg)
+{
+  if (dg->dt_minacc == 0.0) dg->dt_minacc = INFINITY;
+  if (dg->p_min == 0.0) dg->p_min = 1.0;
+}
+
+/* One call of ode_solver(rhs_driver, dt, method) on the whole domain
+ * (solver_library.F90:80-189), methods 0..5; 6 integrates dt adaptively per
+ * column.  ADAPTIVE_EULER is global: any cell anywhere with
+ * c1 - (1 + relative_change_min) c < 0 quarters the step for every column
+ * (:119-129).  last_min_dt (may be NULL) follows :130-135. */
+int oc_ode_solver(const oc_params *p, const oc_domain *d, double *conc,
+                  double dt, int method, double *flux_out, double *cellacc,
+                  double *colacc, oc_coldiag *diag, double *last_min_dt)
+{
+  if (check_domain(d)) return -1;
+  const long ncell = d->ncell, ncol = d->ncol;
+  int rc = 0;
+  if (method == OC_ADAPTIVE_EULER) {
+    double *r = (double *)malloc(sizeof(double) * NV * ncell);
+    double *c1 = (double *)malloc(sizeof(double) * NV * ncell);
+    double *F = (double *)calloc((size_t)NV * ncol, sizeof(double));
+    double *s = (double *)malloc(sizeof(double) * 2 * ncell);
+    double dt_int = 0.0, dt_red = dt;
+    while (dt_int < dt) {
+      for (int c = 0; c < ncol; ++c) {
+        int o = d->off[c], K = d->off[c + 1] - o;
+        if (d->mask[c]) {
+          for (int n = 0; n < NV; ++n) for (int k = 0; k < K; ++k) r[n * ncell + o + k] = 0.0;
+          continue;
+        }
+        colview v;
+        make_view(d, c, &v);
+        double F1[NV];
+        col_rhs(p, &v, conc + o, ncell, r + o, ncell, F1, s + o, s + ncell + o);
+        for (int n = 0; n < NV; ++n) F[n * ncol + c] = F1[n];
+      }
+      int viol = 0;
+      for (long i = 0; i < NV * ncell; ++i) {
+        c1[i] = conc[i] + dt_red * r[i];
+        if ((c1[i] - (1.0 + p->relative_change_min) * conc[i]) < 0.0) viol = 1;
+      }
+      /* masked columns: rhs = 0, so c1 = c and the test is not triggered
+       * for their (positive) missing values */
+      if (viol && dt_red > p->dt_min) {
+        dt_red = dt_red * 0.25;
+      } else {
+        if (last_min_dt && dt_red < *last_min_dt) *last_min_dt = dt_red;
+        for (int c = 0; c < ncol; ++c) {
+          if (d->mask[c]) continue;
+          int o = d->off[c], K = d->off[c + 1] - o;
+          for (int n = 0; n < NV; ++n)
+            for (int k = 0; k < K; ++k) conc[n * ncell + o + k] = c1[n * ncell + o + k];
+          if (colacc) for (int n = 0; n < NV; ++n) colacc[n * ncol + c] += dt_red * F[n * ncol + c];
+          if (cellacc)
+            for (int k = 0; k < K; ++k) {
+              cellacc[o + k] += dt_red * s[o + k];
+              cellacc[ncell + o + k] += dt_red * s[ncell + o + k];
+            }
+          if (diag) {
+            diag[c].accepted += 1;
+            diag[c].dt_last = dt_red;
+            diag_init(&diag[c]);
+            if (dt_red < diag[c].dt_minacc) diag[c].dt_minacc = dt_red;
+          }
+        }
+        dt_int = dt_int + dt_red;
+      }
+    }
+    if (flux_out) memcpy(flux_out, F, sizeof(double) * NV * ncol);
+    if (p->clamp)
+      for (int c = 0; c < ncol; ++c) {
+        if (d->mask[c]) continue;
+        colview v;
+        make_view(d, c, &v);
+        int o = d->off[c];
+        clamp_col(&v, conc + o, ncell, cellacc ? cellacc + o : NULL, ncell);
+      }
+    for (long i = 0; i < NV * ncell; ++i)
+      if (conc[i] != conc[i]) rc = 1;
+    free(r); free(c1); free(F); free(s);
+    return rc;
+  }
+  colwork *w = (colwork *)malloc(sizeof(colwork));
+  for (int c = 0; c < ncol; ++c) {
+    if (d->mask[c]) continue;
+    colview v;
+    make_view(d, c, &v);
+    int o = d->off[c];
+    double F1[NV];
+    oc_coldiag dloc;
+    memset(&dloc, 0, sizeof dloc);
+    oc_coldiag *dg = diag ? &diag[c] : &dloc;
+    diag_init(dg);
+    int r;
+    if (method == OC_EMP_ADAPTIVE)
+      r = col_emp_adaptive(p, &v, conc + o, ncell, dt, F1, cellacc ? cellacc + o : NULL, ncell,
+                           colacc ? colacc + c : NULL, ncol, dg, INT64_MAX, w);
+    else
+      r = col_step(p, &v, method, dt, conc + o, ncell, F1, cellacc ? cellacc + o : NULL, ncell,
+                   colacc ? colacc + c : NULL, ncol, dg, w);
+    if (p->clamp && method <= OC_RK4_38)
+      clamp_col(&v, conc + o, ncell, cellacc ? cellacc + o : NULL, ncell);
+    if (flux_out) for (int n = 0; n < NV; ++n) flux_out[(long)n * ncol + c] = F1[n];
+    if (r) rc = 1;
+  }
+  free(w);
+  return rc;
+}
+
+/* nsteps consecutive ode_solver calls (plus the caller's clamp when
+ * p->clamp), columns in parallel for the column-local methods. */
+int oc_run(const oc_params *p, const oc_domain *d, double *conc, double dt,
+           int nsteps, int method, double *flux_out, double *cellacc,
+           double *colacc, oc_coldiag *diag, int nthreads)
+{
+  if (check_domain(d)) return -1;
+  if (method == OC_ADAPTIVE_EULER) {
+    int rc = 0;
+    for (int s = 0; s < nsteps; ++s)
+      rc |= oc_ode_solver(p, d, conc, dt, method, flux_out, cellacc, colacc, diag, NULL);
+    return rc;
+  }
+  const long ncell = d->ncell, ncol = d->ncol;
+  int rc = 0;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads) reduction(| : rc)
+#endif
+  {
+    colwork *w = (colwork *)malloc(sizeof(colwork));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int c = 0; c < ncol; ++c) {
+      if (d->mask[c]) continue;
+      colview v;
+      make_view(d, c, &v);
+      int o = d->off[c];
+      double F1[NV];
+      oc_coldiag dloc;
+      memset(&dloc, 0, sizeof dloc);
+      oc_coldiag *dg = diag ? &diag[c] : &dloc;
+      diag_init(dg);
+      for (int s = 0; s < nsteps; ++s) {
+        int r = col_step(p, &v, method, dt, conc + o, ncell, F1, cellacc ? cellacc + o : NULL, ncell,
+                         colacc ? colacc + c : NULL, ncol, dg, w);
+        if (p->clamp && method <= OC_RK4_38)
+          clamp_col(&v, conc + o, ncell, cellacc ? cellacc + o : NULL, ncell);
+        if (r) { rc |= 1; dg->status = 1; break; }
+      }
+      if (flux_out) for (int n = 0; n < NV; ++n) flux_out[(long)n * ncol + c] = F1[n];
+    }
+    free(w);
+  }
+  return rc;
+}
+
+/* Adaptive EMP over t_span for every wet column (columns in parallel). */
+int oc_emp_integrate(const oc_params *p, const oc_domain *d, double *conc,
+                     double t_span, double *flux_out, double *cellacc,
+                     double *colacc, oc_coldiag *diag, int64_t max_attempts,
+                     int nthreads)
+{
+  if (check_domain(d)) return -1;
+  const long ncell = d->ncell, ncol = d->ncol;
+  int rc = 0;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads) reduction(| : rc)
+#endif
+  {
+    colwork *w = (colwork *)malloc(sizeof(colwork));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int c = 0; c < ncol; ++c) {
+      if (d->mask[c]) continue;
+      colview v;
+      make_view(d, c, &v);
+      int o = d->off[c];
+      double F1[NV];
+      for (int n = 0; n < NV; ++n) F1[n] = flux_out ? flux_out[(long)n * ncol + c] : 0.0;
+      diag_init(&diag[c]);
+      int r = col_emp_adaptive(p, &v, conc + o, ncell, t_span, F1, cellacc ? cellacc + o : NULL,
+                               ncell, colacc ? colacc + c : NULL, ncol, &diag[c], max_attempts, w);
+      if (flux_out) for (int n = 0; n < NV; ++n) flux_out[(long)n * ncol + c] = F1[n];
+      if (r) rc |= r;
+    }
+    free(w);
+  }
+  return rc;
+}
+
+/* budget[c*OC_NBUD + i]: i<11 integrated top flux per variable, 11 sinkC,
+ * 12 sinkN, 13..15 clamp C/N/P; cell terms weighted by phi*dz and summed
+ * over layers with the pairwise tree. */
+int oc_budget(const oc_domain *d, const double *cellacc, const double *colacc,
+              double *budget)
+{
+  const long ncell = d->ncell, ncol = d->ncol;
+  for (int c = 0; c < ncol; ++c) {
+    int o = d->off[c], K = d->off[c + 1] - o;
+    double *b = budget + (long)c * OC_NBUD;
+    for (int n = 0; n < NV; ++n) b[n] = colacc ? colacc[(long)n * ncol + c] : 0.0;
+    for (int a = 0; a < 5; ++a) {
+      double t[OC_KMAX];
+      for (int k = 0; k < K; ++k) t[k] = (d->phi[o + k] * d->dz[o + k]) * cellacc[a * ncell + o + k];
+      b[NV + a] = tree_sum(t, K);
+    }
+  }
+  return 0;
+}
+
+/* inv[c*NV + n] = sum_k phi_k dz_k c_{k,n} (pairwise tree) */
+int oc_inventory(const oc_domain *d, const double *conc, double *inv)
+{
+  const long ncell = d->ncell;
+  for (int c = 0; c < d->ncol; ++c) {
+    int o = d->off[c], K = d->off[c + 1] - o;
+    for (int n = 0; n < NV; ++n) {
+      double t[OC_KMAX];
+      for (int k = 0; k < K; ++k) t[k] = (d->phi[o + k] * d->dz[o + k]) * conc[n * ncell + o + k];
+      inv[(long)c * NV + n] = tree_sum(t, K);
+    }
+  }
+  return 0;
+}
+
+/* resid[c] = max_{k,n} |rhs_{k,n}| / max(|c_{k,n}|, cfloor[n])  (s-1) */
+int oc_residual(const oc_params *p, const oc_domain *d, const double *conc,
+                const double *cfloor, double *resid)
+{
+  if (check_domain(d)) return -1;
+  const long ncell = d->ncell;
+  double *r = (double *)malloc(sizeof(double) * NV * ncell);
+  oc_rhs(p, d, conc, r, NULL);
+  for (int c = 0; c < d->ncol; ++c) {
+    int o = d->off[c], K = d->off[c + 1] - o;
+    double m = 0.0;
+    if (!d->mask[c])
+      for (int n = 0; n < NV; ++n)
+        for (int k = 0; k < K; ++k) {
+          double x = fabs(conc[n * ncell + o + k]);
+          double den = x > cfloor[n] ? x : cfloor[n];
+          double e = fabs(r[n * ncell + o + k]) / den;
+          if (e > m || e != e) m = e;
+        }
+    resid[c] = m;
+  }
+  free(r);
+  return 0;
+}
