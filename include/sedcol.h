@@ -72,6 +72,16 @@ enum sedcol_status {
 
 enum sedcol_layout { SEDCOL_LAYOUT_FORTRAN = 0, SEDCOL_LAYOUT_PACKED = 1 };
 
+/* Arithmetic form of the right-hand side (sedcol_params.arith).
+ * REFERENCE: the operation order of fabm_sediment_driver.F90:652-692,776-820
+ *            (divisions by dzc, (1-phi)dz and phi as written there).
+ * FAST:      the same mathematics with the geometry divisions folded into
+ *            per-layer coefficients D/dzc and 1/(phi dz) computed once per
+ *            forcing record, and merged Monod quotients; results differ from
+ *            REFERENCE by rounding only.  Requires bcdown != 2 (REFERENCE is
+ *            used otherwise). */
+enum sedcol_arith { SEDCOL_ARITH_REFERENCE = 0, SEDCOL_ARITH_FAST = 1 };
+
 typedef struct sedcol_handle sedcol_handle;
 
 /* Domain description, the counterpart of type_rhs_driver's
@@ -104,6 +114,8 @@ typedef struct {
   int32_t bcdown;             /* 1, 2 or 3 (reference get_rhs uses 3) */
   int32_t clamp;              /* clamp to the minimum 0 after each fixed step, as
                                  fabm_sediment_component.F90:1726-1731 does */
+  int32_t arith;              /* enum sedcol_arith */
+  int32_t reserved;           /* must be 0 */
 } sedcol_params;
 
 /* Per-column diagnostics (sedcol_get_diagnostics), the per-column form of
@@ -113,6 +125,7 @@ typedef struct {
   int64_t rejected;   /* rejected adaptive attempts */
   int64_t forced;     /* accepted at emp_dt_min with error above tolerance */
   int64_t zero_sink;  /* destruction terms of exhausted components (EMP) */
+  int64_t newton;     /* Newton iterations of the Patankar factor (EMP) */
   double dt_last;     /* last accepted step */
   double dt_next;     /* step size carried into the next call */
   double dt_minacc;   /* smallest accepted step */
@@ -157,13 +170,15 @@ int sedcol_set_forcing(sedcol_handle *h, const double *temp, const double *c_bw,
                        const double *flux_top);
 
 /* A piecewise-constant forcing series of nrec records, record r applying on
- * [t0 + r*dt_rec, t0 + (r+1)*dt_rec); arrays are (nrec, ncol),
- * (nrec, nvar, ncol), (nrec, nvar, ncol).  Used by sedcol_integrate.  No
- * reference counterpart: the reference re-reads forcing every coupling step
- * (fabm_sediment_component.F90:1665); this lets one launch span many records. */
+ * [t0 + r*dt_rec, t0 + (r+1)*dt_rec); temp is (nrec, ncol), c_bw
+ * (nrec, nvar, ncol), flux_top (nrec_flux, nvar, ncol) with nrec_flux = nrec
+ * or 1 (one flux record for the whole series).  Used by sedcol_integrate.
+ * No reference counterpart: the reference re-reads forcing every coupling
+ * step (fabm_sediment_component.F90:1665); this lets one launch span many
+ * records. */
 int sedcol_set_forcing_series(sedcol_handle *h, int32_t nrec, double t0, double dt_rec,
                               const double *temp, const double *c_bw,
-                              const double *flux_top);
+                              const double *flux_top, int32_t nrec_flux);
 
 /* State in / out (conc pointer association, fabm_sediment_component.F90:521,
  * :1666-1667; init_concentrations :449-481 is the caller's). */

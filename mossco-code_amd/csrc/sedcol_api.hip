@@ -48,6 +48,7 @@ struct sedcol_handle {
   double *d_dz = nullptr, *d_dzc = nullptr, *d_phi = nullptr, *d_phiI = nullptr, *d_bfac = nullptr;
   double *d_biot = nullptr;
   int nrec = 0;
+  long long fup_rec_stride = 0;
   double t0 = 0.0, dt_rec = INFINITY;
   double *d_temp = nullptr, *d_cbw = nullptr, *d_fup = nullptr;
   size_t forcing_cap = 0;  // records allocated
@@ -83,6 +84,7 @@ struct sedcol_handle {
     d.temp = d_temp;
     d.cbw = d_cbw;
     d.fup = d_fup;
+    d.fup_rec_stride = fup_rec_stride;
     d.conc = d_conc;
     d.cellacc = d_cellacc;
     d.colacc = d_colacc;
@@ -158,7 +160,7 @@ int alloc_forcing(sedcol_handle *h, int nrec)
   HIPOK(hipMalloc(&h->d_temp, sizeof(double) * (size_t)nrec * h->ncol));
   HIPOK(hipMalloc(&h->d_cbw, sizeof(double) * (size_t)nrec * NV * h->ncol));
   HIPOK(hipMalloc(&h->d_fup, sizeof(double) * (size_t)nrec * NV * h->ncol));
-  h->forcing_cap = (size_t)nrec;
+  h->forcing_cap = (size_t)nrec;  // every array sized for nrec records
   return SEDCOL_OK;
 }
 
@@ -272,7 +274,7 @@ int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
   HIPOK_C(hipMalloc(&h->d_flux, sizeof(double) * NV * h->ncol));
   HIPOK_C(hipMalloc(&h->d_diag, sizeof(sedcol_coldiag) * h->ncol));
   HIPOK_C(hipMalloc(&h->d_small, sizeof(double) * (NV + 1) * h->ncol));
-  HIPOK_C(hipMalloc(&h->d_flag, sizeof(int)));
+  HIPOK_C(hipMalloc(&h->d_flag, sizeof(int) * 4));  // error flag + one work-queue counter per bin
   HIPOK_C(hipMemset(h->d_conc, 0, NV * cb));
   HIPOK_C(hipMemset(h->d_cellacc, 0, 5 * cb));
   HIPOK_C(hipMemset(h->d_colacc, 0, sizeof(double) * NV * h->ncol));
@@ -312,6 +314,9 @@ int sedcol_set_params(sedcol_handle *h, const sedcol_params *p)
   if (!(p->emp_dt_min > 0.0) || !(p->emp_dt_max >= p->emp_dt_min))
     return fail(SEDCOL_EINVAL, "need 0 < emp_dt_min <= emp_dt_max");
   if (!(p->emp_rtol >= 0.0) || !(p->emp_atol > 0.0)) return fail(SEDCOL_EINVAL, "need emp_rtol >= 0, emp_atol > 0");
+  if (p->arith != SEDCOL_ARITH_REFERENCE && p->arith != SEDCOL_ARITH_FAST)
+    return fail(SEDCOL_EINVAL, "arith must be SEDCOL_ARITH_REFERENCE or SEDCOL_ARITH_FAST");
+  if (p->reserved != 0) return fail(SEDCOL_EINVAL, "reserved must be 0");
   h->p = *p;
   h->have_params = true;
   return SEDCOL_OK;
@@ -361,20 +366,22 @@ int sedcol_set_geometry(sedcol_handle *h, const double *dz, const double *dzc, c
 }
 
 int sedcol_set_forcing_series(sedcol_handle *h, int32_t nrec, double t0, double dt_rec, const double *temp,
-                              const double *c_bw, const double *flux_top)
+                              const double *c_bw, const double *flux_top, int32_t nrec_flux)
 {
   if (!h || !temp || !c_bw || !flux_top) return fail(SEDCOL_EINVAL, "null argument");
   if (nrec < 1) return fail(SEDCOL_EINVAL, "nrec must be positive");
+  if (nrec_flux != 1 && nrec_flux != nrec) return fail(SEDCOL_EINVAL, "nrec_flux must be 1 or nrec");
   if (nrec > 1 && !(dt_rec > 0.0 && std::isfinite(dt_rec))) return fail(SEDCOL_EINVAL, "dt_rec must be positive");
   if (!std::isfinite(t0)) return fail(SEDCOL_EINVAL, "t0 must be finite");
   int rc = alloc_forcing(h, nrec);
   if (rc) return rc;
   HIPOK(hipMemcpyAsync(h->d_temp, temp, sizeof(double) * (size_t)nrec * h->ncol, hipMemcpyHostToDevice, h->stream));
   HIPOK(hipMemcpyAsync(h->d_cbw, c_bw, sizeof(double) * (size_t)nrec * NV * h->ncol, hipMemcpyHostToDevice, h->stream));
-  HIPOK(hipMemcpyAsync(h->d_fup, flux_top, sizeof(double) * (size_t)nrec * NV * h->ncol, hipMemcpyHostToDevice,
+  HIPOK(hipMemcpyAsync(h->d_fup, flux_top, sizeof(double) * (size_t)nrec_flux * NV * h->ncol, hipMemcpyHostToDevice,
                        h->stream));
   HIPOK(hipStreamSynchronize(h->stream));
   h->nrec = nrec;
+  h->fup_rec_stride = nrec_flux > 1 ? (long long)NV * h->ncol : 0;
   h->t0 = t0;
   h->dt_rec = nrec > 1 ? dt_rec : INFINITY;
   h->have_forcing = true;
@@ -383,7 +390,7 @@ int sedcol_set_forcing_series(sedcol_handle *h, int32_t nrec, double t0, double 
 
 int sedcol_set_forcing(sedcol_handle *h, const double *temp, const double *c_bw, const double *flux_top)
 {
-  return sedcol_set_forcing_series(h, 1, 0.0, INFINITY, temp, c_bw, flux_top);
+  return sedcol_set_forcing_series(h, 1, 0.0, INFINITY, temp, c_bw, flux_top, 1);
 }
 
 int sedcol_set_state(sedcol_handle *h, const double *conc)
@@ -550,7 +557,7 @@ int sedcol_integrate(sedcol_handle *h, double t_start, double t_end)
   if (!std::isfinite(t_start) || !std::isfinite(t_end) || !(t_end >= t_start))
     return fail(SEDCOL_EINVAL, "need finite t_start <= t_end");
   const int r0 = record_of(h, t_start), r1 = record_of(h, t_end);
-  HIPOK(hipMemsetAsync(h->d_flag, 0, sizeof(int), h->stream));
+  HIPOK(hipMemsetAsync(h->d_flag, 0, sizeof(int) * 4, h->stream));
   const Dev d = h->dev();
   int launches = 0;
   const long long max_attempts = 1ll << 40;
@@ -558,7 +565,7 @@ int sedcol_integrate(sedcol_handle *h, double t_start, double t_end)
   for (int b = 0; b < 3; ++b)
     if (!h->bins[b].empty()) {
       sc::launch_emp_adaptive(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, t_start, t_end, r0,
-                              r1, max_attempts, h->d_flag);
+                              r1, max_attempts, h->d_flag + 1 + b, h->d_flag);
       ++launches;
     }
   HIPOK(hipGetLastError());

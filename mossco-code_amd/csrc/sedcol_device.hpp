@@ -2,14 +2,20 @@
 //
 // Execution model: one column per lane group of LPC lanes (LPC = 16, 32 or
 // 64, a power of two >= K), lane l holding layer l with all NV state
-// variables in VGPRs.  Vertical neighbours move between lanes by shuffles;
-// column sums, products and maxima use an xor butterfly, i.e. a balanced
-// pairwise tree whose grouping does not depend on LPC (padding lanes hold the
-// identity).
+// variables in VGPRs.  Vertical neighbours move between lanes with DPP
+// wave shifts; column sums, products and maxima use a butterfly (DPP quad
+// permutes and row mirrors, a 32-lane swizzle, and a bpermute for the
+// 64-lane level), i.e. a balanced pairwise tree whose grouping does not
+// depend on LPC (padding lanes hold the identity).
 //
 // Floating point: built with -ffp-contract=off; fused multiply-adds are
-// explicit fma() calls.  Operation order follows the reference Fortran where
-// the reference defines the arithmetic (cited per function).
+// explicit fma() calls.  Two arithmetic forms exist (template ARITH):
+//   SEDCOL_ARITH_REFERENCE  the operation order of the reference Fortran
+//                           (fabm_sediment_driver.F90:652-692, :776-820);
+//   SEDCOL_ARITH_FAST       geometry divisions replaced by per-lane
+//                           coefficients computed once per forcing record
+//                           (D/dzc, 1/(phi dz)) and merged Monod quotients;
+//                           same mathematics, different rounding.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +26,8 @@
 namespace sc {
 
 constexpr int NV = SEDCOL_NVAR;
+constexpr int ARITH_REF = SEDCOL_ARITH_REFERENCE;
+constexpr int ARITH_FAST = SEDCOL_ARITH_FAST;
 
 // particulate flag per state variable (fabm_sediment_driver.F90:658)
 __host__ __device__ constexpr bool part(int n) { return n <= 4 || n == 10; }
@@ -89,8 +97,9 @@ __device__ __forceinline__ double temp_factor(double T)
 // (fabm_sediment_driver.F90:700); that module is not part of the reference,
 // so this is the library's own definition.  Each destruction term is
 // proportional to the destroyed species.  sC = DIC release, sN = N2 release.
-__device__ __forceinline__ void kinetics(const sedcol_params &p, double fT, const double (&c)[NV],
-                                         double (&R)[NV], double &sC, double &sN)
+template <int ARITH>
+__device__ __forceinline__ void kinetics(const sedcol_params &p, double fT, const double (&c)[NV], double (&R)[NV],
+                                         double &sC, double &sN)
 {
   double ldC = nn(c[0]), sdC = nn(c[1]), ldN = nn(c[2]), sdN = nn(c[3]);
   double dP = nn(c[4]), O2 = nn(c[5]), NO3 = nn(c[6]), NH4 = nn(c[7]);
@@ -109,8 +118,14 @@ __device__ __forceinline__ void kinetics(const sedcol_params &p, double fT, cons
   double Pprod = (kL * dP) * lim;
 
   double oxlim = O2 / (O2 + p.ksO2oxic);
-  double dnlim = (p.kinO2denit / (O2 + p.kinO2denit)) * (NO3 / (NO3 + p.ksNO3denit));
-  double anlim = (p.kinO2anox / (O2 + p.kinO2anox)) * (p.kinNO3anox / (NO3 + p.kinNO3anox));
+  double dnlim, anlim;
+  if (ARITH == ARITH_REF) {
+    dnlim = (p.kinO2denit / (O2 + p.kinO2denit)) * (NO3 / (NO3 + p.ksNO3denit));
+    anlim = (p.kinO2anox / (O2 + p.kinO2anox)) * (p.kinNO3anox / (NO3 + p.kinNO3anox));
+  } else {
+    dnlim = (p.kinO2denit * NO3) / ((O2 + p.kinO2denit) * (NO3 + p.ksNO3denit));
+    anlim = (p.kinO2anox * p.kinNO3anox) / ((O2 + p.kinO2anox) * (NO3 + p.kinNO3anox));
+  }
   double rescale = 1.0 / ((oxlim + dnlim) + anlim);
   double OxicMin = (Cprod * oxlim) * rescale;
   double Denit = (Cprod * dnlim) * rescale;
@@ -139,55 +154,70 @@ __device__ __forceinline__ void kinetics(const sedcol_params &p, double fT, cons
 }
 
 // ---------------------------------------------------------------------------
-// lane-group exchange (groups of LPC lanes inside a wave64)
+// lane exchange
 // ---------------------------------------------------------------------------
 
-// value held by lane l-1 of the group (lane 0: own value, never used)
-template <int LPC>
-__device__ __forceinline__ double lg_up(double v)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
 {
-  return __shfl_up(v, 1u, LPC);
-}
-// value held by lane l+1 of the group (last lane: own value, never used)
-template <int LPC>
-__device__ __forceinline__ double lg_dn(double v)
-{
-  return __shfl_down(v, 1u, LPC);
-}
-template <int LPC>
-__device__ __forceinline__ double lg_bcast0(double v)
-{
-  return __shfl(v, 0, LPC);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
 }
 
-// butterfly sum: every lane receives the balanced-tree sum over the group
-template <int LPC>
+template <int PATTERN>
+__device__ __forceinline__ double swz_d(double v)
+{
+  const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), PATTERN);
+  const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), PATTERN);
+  return __hiloint2double(hi, lo);
+}
+
+// value held by the previous lane (wave_shr:1); lane 0 of a group never uses it
+__device__ __forceinline__ double lg_up(double v) { return dpp_d<0x138>(v); }
+// value held by the next lane (wave_shl:1); the last layer never uses it
+__device__ __forceinline__ double lg_dn(double v) { return dpp_d<0x130>(v); }
+
+// Partner value at butterfly level S (S = 1, 2, 4, ...).  Levels 4 and 8 use
+// row mirrors, which pair the same sub-groups as lane^S once every sub-group
+// of size S holds one value, which the butterfly guarantees because it runs
+// the levels in increasing order.
+template <int S>
+__device__ __forceinline__ double bfly(double v)
+{
+  if constexpr (S == 1) return dpp_d<0xB1>(v);        // quad_perm [1,0,3,2]
+  else if constexpr (S == 2) return dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  else if constexpr (S == 4) return dpp_d<0x141>(v);  // row_half_mirror
+  else if constexpr (S == 8) return dpp_d<0x140>(v);  // row_mirror
+  else if constexpr (S == 16) return swz_d<0x401F>(v); // swizzle xor 16 within 32 lanes
+  else return __shfl_xor(v, 32, 64);
+}
+
+template <int LPC, int S = 1>
 __device__ __forceinline__ double lg_sum(double v)
 {
-#pragma unroll
-  for (int s = 1; s < LPC; s <<= 1) v = v + __shfl_xor(v, s, LPC);
-  return v;
+  if constexpr (S >= LPC) return v;
+  else return lg_sum<LPC, 2 * S>(v + bfly<S>(v));
 }
-template <int LPC>
+template <int LPC, int S = 1>
 __device__ __forceinline__ double lg_max(double v)
 {
-#pragma unroll
-  for (int s = 1; s < LPC; s <<= 1) {
-    double o = __shfl_xor(v, s, LPC);
-    v = o > v ? o : v;
+  if constexpr (S >= LPC) return v;
+  else {
+    const double o = bfly<S>(v);
+    return lg_max<LPC, 2 * S>(o > v ? o : v);
   }
-  return v;
 }
 // (P, D) product-with-derivative pairs: (P1,D1)(P2,D2) = (P1 P2, D1 P2 + P1 D2)
-template <int LPC>
+template <int LPC, int S = 1>
 __device__ __forceinline__ void lg_pd(double &P, double &D)
 {
-#pragma unroll
-  for (int s = 1; s < LPC; s <<= 1) {
-    double Po = __shfl_xor(P, s, LPC);
-    double Do = __shfl_xor(D, s, LPC);
+  if constexpr (S < LPC) {
+    const double Po = bfly<S>(P);
+    const double Do = bfly<S>(D);
     D = D * Po + P * Do;
     P = P * Po;
+    lg_pd<LPC, 2 * S>(P, D);
   }
 }
 
@@ -199,19 +229,33 @@ struct Lane {
   int l;        // layer index (lane in group)
   int K;        // layers of the column
   bool active;  // l < K
-  double dz, dzc, phi;
-  double Dbio, Ddis;  // interface diffusivities for the current temperature
+  double phi;
   double fT;
+  // reference form: geometry and interface diffusivities
+  double dz, dzc, Dbio, Ddis;
+  // fast form: Dbio/dzc, Ddis/dzc (D/dz on the top layer), 1/(phi dz)
+  double Gb, Gd, W;
 };
 
 // Interface diffusivities of get_rhs (fabm_sediment_driver.F90:652-653 and
-// :682-683, division order as written there).
-__device__ __forceinline__ void diffusivities(const sedcol_params &p, double B, double T, double phiI,
-                                             double bfac, Lane &L)
+// :682-683, division order as written there) and, for the fast form, the
+// per-lane transport coefficients derived from them.
+template <int ARITH>
+__device__ __forceinline__ void diffusivities(const sedcol_params &p, double B, double T, double phiI, double bfac,
+                                              Lane &L)
 {
   L.fT = temp_factor(T);
-  L.Dbio = ((((B * L.fT) / 86400.0) / 10000.0) * (1.0 - phiI)) * bfac;
-  L.Ddis = L.Dbio + (((p.D0 + T * 0.035) * phiI) / 86400.0) / 10000.0;
+  const double Dbio = ((((B * L.fT) / 86400.0) / 10000.0) * (1.0 - phiI)) * bfac;
+  const double Ddis = Dbio + (((p.D0 + T * 0.035) * phiI) / 86400.0) / 10000.0;
+  if (ARITH == ARITH_REF) {
+    L.Dbio = Dbio;
+    L.Ddis = Ddis;
+  } else {
+    const double len = L.l == 0 ? L.dz : L.dzc;
+    L.Gb = Dbio / len;
+    L.Gd = Ddis / len;
+    L.W = 1.0 / (L.phi * L.dz);
+  }
 }
 
 // One get_rhs evaluation for the lane's layer.  sbc[0..NV) bottom-water
@@ -220,32 +264,46 @@ __device__ __forceinline__ void diffusivities(const sedcol_params &p, double B, 
 // through the lane's upper interface (lane 0: the top flux intFlux(:,:,1)).
 // Transport: diff3d (fabm_sediment_driver.F90:739-825) with the particulate
 // bulk/volume-fraction handling of get_rhs :658-678 and dissolved :680-692.
-template <int LPC, class CB>
+// The fast form requires bcdown != 2.
+template <int LPC, int ARITH, class CB>
 __device__ __forceinline__ void column_rhs(const sedcol_params &p, const Lane &L, const double *sbc,
                                            const double (&c)[NV], double &sC, double &sN, CB &&cb)
 {
+  // Callbacks may stage values in LDS for other lanes of the same wave; the
+  // wave barriers keep those accesses ordered around the evaluation.
+  __builtin_amdgcn_wave_barrier();
   double R[NV];
-  kinetics(p, L.fT, c, R, sC, sN);
+  kinetics<ARITH>(p, L.fT, c, R, sC, sN);
+  const bool top = L.l == 0;
+  const bool bottom = L.l == L.K - 1;
 #pragma unroll
   for (int n = 0; n < NV; ++n) {
     const bool pt = part(n);
     const double x = pt ? c[n] * L.phi : c[n];
-    const double xup = lg_up<LPC>(x);
-    const double D = pt ? L.Dbio : L.Ddis;
+    const double xup = lg_up(x);
+    const int bc = pt ? p.bcup_part : p.bcup_dis;
     double F;
-    if (L.l == 0) {
-      const int bc = pt ? p.bcup_part : p.bcup_dis;
-      F = bc == 1 ? sbc[NV + n] : bc == 2 ? -((D * (x - sbc[n])) / L.dz) : 0.0;
+    if (ARITH == ARITH_REF) {
+      const double D = pt ? L.Dbio : L.Ddis;
+      if (top) F = bc == 1 ? sbc[NV + n] : bc == 2 ? -((D * (x - sbc[n])) / L.dz) : 0.0;
+      else F = -((D * (x - xup)) / L.dzc);
+      double Fdn = lg_dn(F);
+      if (bottom) Fdn = p.bcdown == 2 ? -((D * (0.0 - x)) / L.dz) : 0.0;
+      const double VF = pt ? (1.0 - L.phi) : L.phi;
+      const double dC = (F - Fdn) / (VF * L.dz);
+      const double Q = pt ? (dC * (1.0 - L.phi)) / L.phi : dC;
+      cb(n, R[n] + Q, F);
     } else {
-      F = -((D * (x - xup)) / L.dzc);
+      const double G = pt ? L.Gb : L.Gd;
+      if (top) F = bc == 1 ? sbc[NV + n] : bc == 2 ? -(G * (x - sbc[n])) : 0.0;
+      else F = -(G * (x - xup));
+      double Fdn = lg_dn(F);
+      if (bottom) Fdn = 0.0;
+      const double Q = (F - Fdn) * L.W;
+      cb(n, R[n] + Q, F);
     }
-    double Fdn = lg_dn<LPC>(F);
-    if (L.l == L.K - 1) Fdn = p.bcdown == 2 ? -((D * (0.0 - x)) / L.dz) : 0.0;
-    const double VF = pt ? (1.0 - L.phi) : L.phi;
-    const double dC = (F - Fdn) / (VF * L.dz);
-    const double Q = pt ? (dC * (1.0 - L.phi)) / L.phi : dC;
-    cb(n, R[n] + Q, F);
   }
+  __builtin_amdgcn_wave_barrier();
 }
 
 // Patankar factor of one column: root p in (0,1] of
@@ -253,7 +311,7 @@ __device__ __forceinline__ void column_rhs(const sedcol_params &p, const Lane &L
 // from p = 0 on the convex decreasing h(p) = prod - p.
 template <int LPC>
 __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g)[NV], double h, bool active,
-                                         int64_t &zero_sink)
+                                         int64_t &zero_sink, int64_t &newton)
 {
   double a[NV];
 #pragma unroll
@@ -267,6 +325,7 @@ __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g
   }
   double pp = 0.0;
   for (int it = 0; it < 60; ++it) {
+    newton += 1;
     double P = 1.0, D = 0.0;
 #pragma unroll
     for (int n = 0; n < NV; ++n) {

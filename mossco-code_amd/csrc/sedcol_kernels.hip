@@ -6,6 +6,10 @@
 // half its lanes.  Fixed-step methods keep the state in registers for a
 // whole window of steps; the adaptive EMP kernel keeps it for the whole
 // integration span and every forcing record inside it.
+//
+// Top fluxes are produced on lane 0 (one value per variable and stage); they
+// are staged in LDS and accumulated by lanes 0..NV-1, lane n owning
+// variable n, so no lane carries per-variable flux registers.
 #include "sedcol_kernels.hpp"
 
 namespace sc {
@@ -20,7 +24,6 @@ struct Geo {
   double phiI, bfac, B;
 };
 
-template <int LPC>
 __device__ __forceinline__ void setup_lane(const Dev &d, int c, int l, Lane &L, Geo &G, long long &cell)
 {
   const int o = d.off[c];
@@ -37,17 +40,17 @@ __device__ __forceinline__ void setup_lane(const Dev &d, int c, int l, Lane &L, 
 }
 
 // load forcing record r of column c: bottom-water concentrations and top
-// fluxes to the group's LDS slot, temperature-dependent diffusivities to L
-template <int LPC>
+// fluxes to the group's LDS slot, temperature-dependent coefficients to L
+template <int ARITH>
 __device__ __forceinline__ void load_forcing(const Dev &d, const sedcol_params &p, int c, int r, const Geo &G,
                                              Lane &L, double *sbc)
 {
   if (L.l < NV) {
     sbc[L.l] = d.cbw[((long long)r * NV + L.l) * d.ncol + c];
-    sbc[NV + L.l] = d.fup[((long long)r * NV + L.l) * d.ncol + c];
+    sbc[NV + L.l] = d.fup[(long long)r * d.fup_rec_stride + (long long)L.l * d.ncol + c];
   }
   const double T = d.temp[(long long)r * d.ncol + c];
-  diffusivities(p, G.B, T, G.phiI, G.bfac, L);
+  diffusivities<ARITH>(p, G.B, T, G.phiI, G.bfac, L);
 }
 
 template <int LPC>
@@ -60,7 +63,7 @@ __device__ __forceinline__ bool group_any(bool b)
 // single right-hand side (get_rhs) for parity and diagnostics
 // ---------------------------------------------------------------------------
 
-template <int LPC>
+template <int LPC, int ARITH>
 __global__ __launch_bounds__(BLOCK) void k_rhs(Dev d, const int *list, int nlist, sedcol_params p, int rec,
                                                const double *conc, double *rhs, double *flux)
 {
@@ -74,8 +77,8 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(Dev d, const int *list, int nlist
   Geo Gm;
   long long cell = 0;
   if (valid) {
-    setup_lane<LPC>(d, c, l, L, Gm, cell);
-    load_forcing<LPC>(d, p, c, rec, Gm, L, sbc[g]);
+    setup_lane(d, c, l, L, Gm, cell);
+    load_forcing<ARITH>(d, p, c, rec, Gm, L, sbc[g]);
   }
   __syncthreads();
   if (!valid) return;
@@ -83,7 +86,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(Dev d, const int *list, int nlist
 #pragma unroll
   for (int n = 0; n < NV; ++n) x[n] = L.active ? conc[n * d.ncell + cell] : 0.0;
   double sC, sN;
-  column_rhs<LPC>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
+  column_rhs<LPC, ARITH>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
     if (L.active) rhs[n * d.ncell + cell] = r;
     if (l == 0 && flux) flux[(long long)n * d.ncol + c] = F;
   });
@@ -93,12 +96,13 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(Dev d, const int *list, int nlist
 // fixed-step window: EULER, RK4, RK4 3/8, EMP-1, EMP-2
 // ---------------------------------------------------------------------------
 
-template <int LPC, int METHOD>
+template <int LPC, int METHOD, int ARITH>
 __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nlist, sedcol_params p, int rec,
                                                  double dt, int nsteps, int *errflag)
 {
   constexpr int G = BLOCK / LPC;
   __shared__ double sbc[G][2 * NV];
+  __shared__ double sF[G][4][NV];
   const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
   const int slot = blockIdx.x * G + g;
   const bool valid = slot < nlist;
@@ -107,20 +111,21 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
   Geo Gm;
   long long cell = 0;
   if (valid) {
-    setup_lane<LPC>(d, c, l, L, Gm, cell);
-    load_forcing<LPC>(d, p, c, rec, Gm, L, sbc[g]);
+    setup_lane(d, c, l, L, Gm, cell);
+    load_forcing<ARITH>(d, p, c, rec, Gm, L, sbc[g]);
   }
   __syncthreads();
   if (!valid) return;
   const double *bc = sbc[g];
+  double(*F)[NV] = sF[g];
   const bool act = L.active;
+  const bool own = l < NV;  // lane l accumulates the top flux of variable l
 
-  double x[NV], fint[NV];
+  double x[NV];
 #pragma unroll
-  for (int n = 0; n < NV; ++n) {
-    x[n] = act ? d.conc[n * d.ncell + cell] : 0.0;
-    fint[n] = l == 0 ? d.colacc[(long long)n * d.ncol + c] : 0.0;
-  }
+  for (int n = 0; n < NV; ++n) x[n] = act ? d.conc[n * d.ncell + cell] : 0.0;
+  double fint = own ? d.colacc[(long long)l * d.ncol + c] : 0.0;
+  double flast = own ? d.flux[(long long)l * d.ncol + c] : 0.0;
   double acc[5];
 #pragma unroll
   for (int a = 0; a < 5; ++a) acc[a] = act ? d.cellacc[a * d.ncell + cell] : 0.0;
@@ -128,22 +133,19 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
   int64_t zs = 0;
   const double third = 1.0 / 3.0;
 
-  int s = 0;
-  for (; s < nsteps; ++s) {
-    const bool last = s == nsteps - 1;
-    double fc[NV];
+  for (int s = 0; s < nsteps; ++s) {
     double sC, sN;
     if (METHOD == SEDCOL_EULER) {
       double r0[NV];
-      column_rhs<LPC>(p, L, bc, x, sC, sN, [&](int n, double r, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, x, sC, sN, [&](int n, double r, double Fv) {
         r0[n] = r;
-        fc[n] = F;
-        if (last && l == 0) d.flux[(long long)n * d.ncol + c] = F;
+        if (l == 0) F[0][n] = Fv;
       });
 #pragma unroll
-      for (int n = 0; n < NV; ++n) {
-        x[n] = x[n] + dt * r0[n];
-        fint[n] += dt * fc[n];
+      for (int n = 0; n < NV; ++n) x[n] = x[n] + dt * r0[n];
+      if (own) {
+        fint += dt * F[0][l];
+        flast = F[0][l];
       }
       acc[0] += dt * sC;
       acc[1] += dt * sN;
@@ -151,96 +153,96 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
       // solver_library.F90:142-163
       double cs[NV], k[NV];
       double aC, aN;
-      column_rhs<LPC>(p, L, bc, x, sC, sN, [&](int n, double r, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, x, sC, sN, [&](int n, double r, double Fv) {
         k[n] = 0.5 * r;
         cs[n] = x[n] + (0.5 * dt) * r;
-        fc[n] = 0.5 * F;
+        if (l == 0) F[0][n] = Fv;
       });
       aC = 0.5 * sC;
       aN = 0.5 * sN;
-      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         k[n] = k[n] + r;
         cs[n] = x[n] + (0.5 * dt) * r;
-        fc[n] = fc[n] + F;
+        if (l == 0) F[1][n] = Fv;
       });
       aC = aC + sC;
       aN = aN + sN;
-      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         k[n] = k[n] + r;
         cs[n] = x[n] + dt * r;
-        fc[n] = fc[n] + F;
+        if (l == 0) F[2][n] = Fv;
       });
       aC = aC + sC;
       aN = aN + sN;
-      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         k[n] = k[n] + 0.5 * r;
-        fc[n] = fc[n] + 0.5 * F;
-        if (last && l == 0) d.flux[(long long)n * d.ncol + c] = F;
+        if (l == 0) F[3][n] = Fv;
       });
       aC = aC + 0.5 * sC;
       aN = aN + 0.5 * sN;
       const double w = dt * third;
 #pragma unroll
-      for (int n = 0; n < NV; ++n) {
-        x[n] = x[n] + w * k[n];
-        fint[n] += w * fc[n];
+      for (int n = 0; n < NV; ++n) x[n] = x[n] + w * k[n];
+      if (own) {
+        fint += w * (((0.5 * F[0][l] + F[1][l]) + F[2][l]) + 0.5 * F[3][l]);
+        flast = F[3][l];
       }
       acc[0] += w * aC;
       acc[1] += w * aN;
     } else if (METHOD == SEDCOL_RUNGE_KUTTA_4_38) {
       // solver_library.F90:164-185
       double cs[NV], k1[NV], k2[NV];
-      double f1[NV], f2[NV], s1C, s1N, s2C, s2N;
-      column_rhs<LPC>(p, L, bc, x, sC, sN, [&](int n, double r, double F) {
+      double s1C, s1N, s2C, s2N;
+      column_rhs<LPC, ARITH>(p, L, bc, x, sC, sN, [&](int n, double r, double Fv) {
         k1[n] = r;
         cs[n] = x[n] + (third * dt) * r;
-        f1[n] = F;
+        if (l == 0) F[0][n] = Fv;
       });
       s1C = sC;
       s1N = sN;
-      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         k2[n] = r;
         cs[n] = x[n] + dt * (r - third * k1[n]);
-        f2[n] = F;
+        if (l == 0) F[1][n] = Fv;
       });
       s2C = sC;
       s2N = sN;
-      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         cs[n] = x[n] + dt * ((k1[n] - k2[n]) + r);
         k1[n] = (k1[n] + 3.0 * k2[n]) + 3.0 * r;
-        fc[n] = (f1[n] + 3.0 * f2[n]) + 3.0 * F;
+        if (l == 0) F[2][n] = Fv;
       });
       double aC = (s1C + 3.0 * s2C) + 3.0 * sC;
       double aN = (s1N + 3.0 * s2N) + 3.0 * sN;
-      column_rhs<LPC>(p, L, bc, cs, sC, sN, [&](int n, double r, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         k1[n] = k1[n] + r;
-        fc[n] = fc[n] + F;
-        if (last && l == 0) d.flux[(long long)n * d.ncol + c] = F;
+        if (l == 0) F[3][n] = Fv;
       });
       aC = aC + sC;
       aN = aN + sN;
       const double w = (dt * 1.0) / 8.0;
 #pragma unroll
-      for (int n = 0; n < NV; ++n) {
-        x[n] = x[n] + w * k1[n];
-        fint[n] += w * fc[n];
+      for (int n = 0; n < NV; ++n) x[n] = x[n] + w * k1[n];
+      if (own) {
+        fint += w * (((F[0][l] + 3.0 * F[1][l]) + 3.0 * F[2][l]) + F[3][l]);
+        flast = F[3][l];
       }
       acc[0] += w * aC;
       acc[1] += w * aN;
     } else {  // EMP-1 / EMP-2 with a fixed step
       double f0[NV];
-      column_rhs<LPC>(p, L, bc, x, sC, sN, [&](int n, double r, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, x, sC, sN, [&](int n, double r, double Fv) {
         f0[n] = r;
-        fc[n] = F;
-        if (METHOD == SEDCOL_EMP1 && last && l == 0) d.flux[(long long)n * d.ncol + c] = F;
+        if (l == 0) F[0][n] = Fv;
       });
-      const double p1 = psolve<LPC>(x, f0, dt, act, zs);
+      const double p1 = psolve<LPC>(x, f0, dt, act, zs, dg.newton);
       if (p1 < dg.p_min) dg.p_min = p1;
       if (METHOD == SEDCOL_EMP1) {
 #pragma unroll
-        for (int n = 0; n < NV; ++n) {
-          x[n] = x[n] + (dt * p1) * f0[n];
-          fint[n] += (dt * p1) * fc[n];
+        for (int n = 0; n < NV; ++n) x[n] = x[n] + (dt * p1) * f0[n];
+        if (own) {
+          fint += (dt * p1) * F[0][l];
+          flast = F[0][l];
         }
         acc[0] += (dt * p1) * sC;
         acc[1] += (dt * p1) * sN;
@@ -252,17 +254,17 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
           c1[n] = y < 0.0 ? 0.0 : y;
         }
         const double s0C = sC, s0N = sN;
-        column_rhs<LPC>(p, L, bc, c1, sC, sN, [&](int n, double r, double F) {
+        column_rhs<LPC, ARITH>(p, L, bc, c1, sC, sN, [&](int n, double r, double Fv) {
           f0[n] = (f0[n] + r) * 0.5;
-          fc[n] = (fc[n] + F) * 0.5;
-          if (last && l == 0) d.flux[(long long)n * d.ncol + c] = F;
+          if (l == 0) F[1][n] = Fv;
         });
-        const double p2 = psolve<LPC>(x, f0, dt, act, zs);
+        const double p2 = psolve<LPC>(x, f0, dt, act, zs, dg.newton);
         if (p2 < dg.p_min) dg.p_min = p2;
 #pragma unroll
-        for (int n = 0; n < NV; ++n) {
-          x[n] = x[n] + (dt * p2) * f0[n];
-          fint[n] += (dt * p2) * fc[n];
+        for (int n = 0; n < NV; ++n) x[n] = x[n] + (dt * p2) * f0[n];
+        if (own) {
+          fint += (dt * p2) * ((F[0][l] + F[1][l]) * 0.5);
+          flast = F[1][l];
         }
         acc[0] += (dt * p2) * ((s0C + sC) * 0.5);
         acc[1] += (dt * p2) * ((s0N + sN) * 0.5);
@@ -291,9 +293,11 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
   }
 
 #pragma unroll
-  for (int n = 0; n < NV; ++n) {
+  for (int n = 0; n < NV; ++n)
     if (act) d.conc[n * d.ncell + cell] = x[n];
-    if (l == 0) d.colacc[(long long)n * d.ncol + c] = fint[n];
+  if (own) {
+    d.colacc[(long long)l * d.ncol + c] = fint;
+    d.flux[(long long)l * d.ncol + c] = flast;
   }
 #pragma unroll
   for (int a = 0; a < 5; ++a)
@@ -309,84 +313,119 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 // adaptive EMP: per-column step size, persistent over [t_start, t_end)
 // ---------------------------------------------------------------------------
 
-template <int LPC>
-__global__ __launch_bounds__(BLOCK) void k_emp_adaptive(Dev d, const int *list, int nlist, sedcol_params p,
-                                                        double t_start, double t_end, int r0, int r1,
-                                                        long long max_attempts, int *errflag)
+// minimum waves per SIMD requested from the register allocator: 2 fits the
+// adaptive step in 256 VGPRs without scratch (1 lets it take 256 + AGPRs,
+// 3 spills about 340 bytes per lane)
+#ifndef SEDCOL_EMP_MIN_WAVES
+#define SEDCOL_EMP_MIN_WAVES 2
+#endif
+
+// Work-queue kernel: a grid sized to the resident capacity, each lane group
+// taking the next column from *next (atomic counter) when it finishes one,
+// so columns needing many steps do not leave other lanes idle.  Every column
+// is integrated over [t_start, t_end) record by record (record r covers
+// [t0 + r dt_rec, t0 + (r+1) dt_rec), the first and last extend outward),
+// with its own step size carried in diag.dt_next.  Each group exits when the
+// queue is empty, so every wave reaches the end.
+template <int LPC, int ARITH>
+__global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(Dev d, const int *list, int nlist,
+                                                                             sedcol_params p, double t_start,
+                                                                             double t_end, int r0, int r1,
+                                                                             long long max_attempts, int *next,
+                                                                             int *errflag)
 {
   constexpr int G = BLOCK / LPC;
   __shared__ double sbc[G][2 * NV];
-  __shared__ double sF[G][NV];
-  __shared__ double sFl[G][NV];
+  __shared__ double sF[G][2][NV];
+  __shared__ int sslot[G];
   const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
-  const int slot = blockIdx.x * G + g;
-  const bool valid = slot < nlist;
-  const int c = valid ? list[slot] : 0;
+  const bool own = l < NV;  // lane l accumulates the top flux of variable l
+  double(*F)[NV] = sF[g];
+  const double *bc = sbc[g];
+
+  // state of the group's current column
+  int c = 0, r = 0;
   Lane L;
   Geo Gm;
   long long cell = 0;
-  double x[NV], fint[NV], acc[5];
+  double x[NV];
+  double sinkC = 0.0, sinkN = 0.0, fint = 0.0, flast = 0.0;
   sedcol_coldiag dg;
   int64_t zs = 0;
-  bool dead = !valid;
-  double dt = 0.0;
-  if (valid) {
-    setup_lane<LPC>(d, c, l, L, Gm, cell);
-#pragma unroll
-    for (int n = 0; n < NV; ++n) {
-      x[n] = L.active ? d.conc[n * d.ncell + cell] : 0.0;
-      fint[n] = l == 0 ? d.colacc[(long long)n * d.ncol + c] : 0.0;
-    }
-#pragma unroll
-    for (int a = 0; a < 5; ++a) acc[a] = L.active ? d.cellacc[a * d.ncell + cell] : 0.0;
-    dg = d.diag[c];
-    dt = dg.dt_next;
-    if (!(dt > 0.0)) dt = p.emp_dt_max;
-    if (l < NV) sFl[g][l] = d.flux[(long long)l * d.ncol + c];
-  }
-  const bool act = valid && L.active;
   long long attempts = 0;
+  double dt = 0.0, rem = 0.0;
+  bool have = false, act = false;
 
-  for (int r = r0; r <= r1; ++r) {
-    const double lo = (r == 0) ? -INFINITY : d.t0 + (double)r * d.dt_rec;
-    const double hi = (r == d.nrec - 1) ? INFINITY : d.t0 + (double)(r + 1) * d.dt_rec;
-    const double a = t_start > lo ? t_start : lo;
-    const double b = t_end < hi ? t_end : hi;
-    const double span = b - a;
-    __syncthreads();
-    if (valid) load_forcing<LPC>(d, p, c, r, Gm, L, sbc[g]);
-    __syncthreads();
-    if (dead || !(span > 0.0)) continue;
-    const double *bc = sbc[g];
-    double rem = span;
-    while (rem > 0.0) {
+  for (;;) {
+    if (!have) {
+      if (l == 0) sslot[g] = atomicAdd(next, 1);
+      __builtin_amdgcn_wave_barrier();
+      const int s = sslot[g];
+      __builtin_amdgcn_wave_barrier();
+      if (s >= nlist) break;
+      c = list[s];
+      setup_lane(d, c, l, L, Gm, cell);
+      act = L.active;
+#pragma unroll
+      for (int n = 0; n < NV; ++n) x[n] = act ? d.conc[n * d.ncell + cell] : 0.0;
+      sinkC = act ? d.cellacc[cell] : 0.0;
+      sinkN = act ? d.cellacc[d.ncell + cell] : 0.0;
+      fint = own ? d.colacc[(long long)l * d.ncol + c] : 0.0;
+      flast = own ? d.flux[(long long)l * d.ncol + c] : 0.0;
+      dg = d.diag[c];
+      dt = dg.dt_next;
+      if (!(dt > 0.0)) dt = p.emp_dt_max;
+      zs = 0;
+      attempts = 0;
+      r = r0 - 1;
+      rem = 0.0;
+      have = true;
+    }
+    bool finish = false;
+    if (!(rem > 0.0)) {
+      // next forcing record with a positive span, or the column is done
+      ++r;
+      if (r > r1) {
+        finish = true;
+      } else {
+        const double lo = (r == 0) ? -INFINITY : d.t0 + (double)r * d.dt_rec;
+        const double hi = (r == d.nrec - 1) ? INFINITY : d.t0 + (double)(r + 1) * d.dt_rec;
+        const double a = t_start > lo ? t_start : lo;
+        const double b = t_end < hi ? t_end : hi;
+        rem = b - a;
+        load_forcing<ARITH>(d, p, c, r, Gm, L, sbc[g]);
+        __builtin_amdgcn_wave_barrier();
+        continue;
+      }
+    }
+    if (!finish) {
       if (attempts >= max_attempts) {
         dg.status = 2;
-        dead = true;
         if (l == 0) atomicOr(errflag, 2);
-        break;
+        finish = true;
       }
+    }
+    if (!finish) {
       ++attempts;
       const bool clipped = !(dt < rem);
       const double h = clipped ? rem : dt;
-      double f0[NV], fc[NV], c1[NV];
+      double f0[NV], c1[NV];
       double s0C, s0N, s1C, s1N;
-      column_rhs<LPC>(p, L, bc, x, s0C, s0N, [&](int n, double rr, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, x, s0C, s0N, [&](int n, double rr, double Fv) {
         f0[n] = rr;
-        fc[n] = F;
+        if (l == 0) F[0][n] = Fv;
       });
-      const double p1 = psolve<LPC>(x, f0, h, act, zs);
+      const double p1 = psolve<LPC>(x, f0, h, act, zs, dg.newton);
 #pragma unroll
       for (int n = 0; n < NV; ++n) {
         const double y = x[n] + (h * p1) * f0[n];
         c1[n] = y < 0.0 ? 0.0 : y;
       }
-      column_rhs<LPC>(p, L, bc, c1, s1C, s1N, [&](int n, double rr, double F) {
+      column_rhs<LPC, ARITH>(p, L, bc, c1, s1C, s1N, [&](int n, double rr, double Fv) {
         f0[n] = (f0[n] + rr) * 0.5;
-        fc[n] = (fc[n] + F) * 0.5;
-        if (l == 0) sF[g][n] = F;
+        if (l == 0) F[1][n] = Fv;
       });
-      const double p2 = psolve<LPC>(x, f0, h, act, zs);
+      const double p2 = psolve<LPC>(x, f0, h, act, zs, dg.newton);
       double el = 0.0;
       bool bad = false;
 #pragma unroll
@@ -403,60 +442,68 @@ __global__ __launch_bounds__(BLOCK) void k_emp_adaptive(Dev d, const int *list, 
       const double err = lg_max<LPC>(el);
       if (group_any<LPC>(bad)) {
         dg.status = 1;
-        dead = true;
         if (l == 0) atomicOr(errflag, 1);
-        break;
-      }
-      const bool accept = (err <= 1.0) || !(h > p.emp_dt_min);
-      double fac = err > 0.0 ? 0.9 * drsqrt(err) : 5.0;
-      if (fac < 0.2) fac = 0.2;
-      if (fac > 5.0) fac = 5.0;
-      if (p1 < dg.p_min) dg.p_min = p1;
-      if (p2 < dg.p_min) dg.p_min = p2;
-      if (accept) {
-#pragma unroll
-        for (int n = 0; n < NV; ++n) {
-          double y = x[n] + (h * p2) * f0[n];
-          if (y < 0.0) {
-            if (elem(n) >= 0) acc[2 + elem(n)] += 0.0 - y;
-            y = 0.0;
-          }
-          x[n] = y;
-          fint[n] += (h * p2) * fc[n];
-        }
-        acc[0] += (h * p2) * ((s0C + s1C) * 0.5);
-        acc[1] += (h * p2) * ((s0N + s1N) * 0.5);
-        if (l < NV) sFl[g][l] = sF[g][l];
-        rem = rem - h;
-        dg.accepted += 1;
-        if (err > 1.0) dg.forced += 1;
-        dg.dt_last = h;
-        if (h < dg.dt_minacc) dg.dt_minacc = h;
-        const double prop = h * fac;
-        dt = (clipped && dt > prop) ? dt : prop;
+        finish = true;
       } else {
-        dg.rejected += 1;
-        dt = h * fac;
+        const bool accept = (err <= 1.0) || !(h > p.emp_dt_min);
+        double fac = err > 0.0 ? 0.9 * drsqrt(err) : 5.0;
+        if (fac < 0.2) fac = 0.2;
+        if (fac > 5.0) fac = 5.0;
+        if (p1 < dg.p_min) dg.p_min = p1;
+        if (p2 < dg.p_min) dg.p_min = p2;
+        if (accept) {
+#pragma unroll
+          for (int n = 0; n < NV; ++n) {
+            double y = x[n] + (h * p2) * f0[n];
+            if (y < 0.0) {
+              // rounding guard; rare, so accumulated straight into the
+              // lane's own cell of the budget array
+              if (elem(n) >= 0 && act) d.cellacc[(2 + elem(n)) * d.ncell + cell] += 0.0 - y;
+              y = 0.0;
+            }
+            x[n] = y;
+          }
+          if (own) {
+            fint += (h * p2) * ((F[0][l] + F[1][l]) * 0.5);
+            flast = F[1][l];
+          }
+          sinkC += (h * p2) * ((s0C + s1C) * 0.5);
+          sinkN += (h * p2) * ((s0N + s1N) * 0.5);
+          rem = rem - h;
+          dg.accepted += 1;
+          if (err > 1.0) dg.forced += 1;
+          dg.dt_last = h;
+          if (h < dg.dt_minacc) dg.dt_minacc = h;
+          const double prop = h * fac;
+          dt = (clipped && dt > prop) ? dt : prop;
+        } else {
+          dg.rejected += 1;
+          dt = h * fac;
+        }
+        if (dt < p.emp_dt_min) dt = p.emp_dt_min;
+        if (dt > p.emp_dt_max) dt = p.emp_dt_max;
       }
-      if (dt < p.emp_dt_min) dt = p.emp_dt_min;
-      if (dt > p.emp_dt_max) dt = p.emp_dt_max;
     }
-  }
-  if (!valid) return;
+    if (finish) {
 #pragma unroll
-  for (int n = 0; n < NV; ++n) {
-    if (act) d.conc[n * d.ncell + cell] = x[n];
-    if (l == 0) d.colacc[(long long)n * d.ncol + c] = fint[n];
-  }
-#pragma unroll
-  for (int a = 0; a < 5; ++a)
-    if (act) d.cellacc[a * d.ncell + cell] = acc[a];
-  if (l < NV) d.flux[(long long)l * d.ncol + c] = sFl[g][l];
-  const double zsum = lg_sum<LPC>((double)zs);
-  if (l == 0) {
-    dg.zero_sink += (int64_t)zsum;
-    dg.dt_next = dt;
-    d.diag[c] = dg;
+      for (int n = 0; n < NV; ++n)
+        if (act) d.conc[n * d.ncell + cell] = x[n];
+      if (act) {
+        d.cellacc[cell] = sinkC;
+        d.cellacc[d.ncell + cell] = sinkN;
+      }
+      if (own) {
+        d.colacc[(long long)l * d.ncol + c] = fint;
+        d.flux[(long long)l * d.ncol + c] = flast;
+      }
+      const double zsum = lg_sum<LPC>((double)zs);
+      if (l == 0) {
+        dg.zero_sink += (int64_t)zsum;
+        dg.dt_next = dt;
+        d.diag[c] = dg;
+      }
+      have = false;
+    }
   }
 }
 
@@ -465,7 +512,7 @@ __global__ __launch_bounds__(BLOCK) void k_emp_adaptive(Dev d, const int *list, 
 // each sub-step is a right-hand side, a domain-wide check and a commit
 // ---------------------------------------------------------------------------
 
-template <int LPC>
+template <int LPC, int ARITH>
 __global__ __launch_bounds__(BLOCK) void k_rhs_store(Dev d, const int *list, int nlist, sedcol_params p, int rec,
                                                      double *r_out, double *s_out)
 {
@@ -479,8 +526,8 @@ __global__ __launch_bounds__(BLOCK) void k_rhs_store(Dev d, const int *list, int
   Geo Gm;
   long long cell = 0;
   if (valid) {
-    setup_lane<LPC>(d, c, l, L, Gm, cell);
-    load_forcing<LPC>(d, p, c, rec, Gm, L, sbc[g]);
+    setup_lane(d, c, l, L, Gm, cell);
+    load_forcing<ARITH>(d, p, c, rec, Gm, L, sbc[g]);
   }
   __syncthreads();
   if (!valid) return;
@@ -488,7 +535,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs_store(Dev d, const int *list, int
 #pragma unroll
   for (int n = 0; n < NV; ++n) x[n] = L.active ? d.conc[n * d.ncell + cell] : 0.0;
   double sC, sN;
-  column_rhs<LPC>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
+  column_rhs<LPC, ARITH>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
     if (L.active) r_out[n * d.ncell + cell] = r;
     if (l == 0) d.flux[(long long)n * d.ncol + c] = F;
   });
@@ -530,9 +577,8 @@ __global__ __launch_bounds__(BLOCK) void k_ae_commit(Dev d, const int *list, int
   if (slot >= nlist) return;
   const int c = list[slot];
   const int o = d.off[c], K = d.off[c + 1] - o;
+  if (l < NV) d.colacc[(long long)l * d.ncol + c] += dt_red * d.flux[(long long)l * d.ncol + c];
   if (l == 0) {
-#pragma unroll
-    for (int n = 0; n < NV; ++n) d.colacc[(long long)n * d.ncol + c] += dt_red * d.flux[(long long)n * d.ncol + c];
     sedcol_coldiag dg = d.diag[c];
     dg.accepted += 1;
     dg.dt_last = dt_red;
@@ -618,7 +664,7 @@ __global__ __launch_bounds__(BLOCK) void k_inventory(Dev d, const int *list, int
   }
 }
 
-template <int LPC>
+template <int LPC, int ARITH>
 __global__ __launch_bounds__(BLOCK) void k_residual(Dev d, const int *list, int nlist, sedcol_params p, int rec,
                                                     const double *cfloor, double *resid)
 {
@@ -632,8 +678,8 @@ __global__ __launch_bounds__(BLOCK) void k_residual(Dev d, const int *list, int 
   Geo Gm;
   long long cell = 0;
   if (valid) {
-    setup_lane<LPC>(d, c, l, L, Gm, cell);
-    load_forcing<LPC>(d, p, c, rec, Gm, L, sbc[g]);
+    setup_lane(d, c, l, L, Gm, cell);
+    load_forcing<ARITH>(d, p, c, rec, Gm, L, sbc[g]);
   }
   __syncthreads();
   if (!valid) return;
@@ -641,7 +687,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual(Dev d, const int *list, int 
 #pragma unroll
   for (int n = 0; n < NV; ++n) x[n] = L.active ? d.conc[n * d.ncell + cell] : 0.0;
   double sC, sN, m = 0.0;
-  column_rhs<LPC>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
+  column_rhs<LPC, ARITH>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
     const double ax = fabs(x[n]);
     const double den = ax > cfloor[n] ? ax : cfloor[n];
     const double e = fabs(r) / den;
@@ -662,26 +708,31 @@ static inline dim3 grid_for(int n)
   return dim3((unsigned)((n + G - 1) / G));
 }
 
-#define SC_DISPATCH_LPC(lpc, ...)                   \
-  switch (lpc) {                                    \
-    case 16: { constexpr int LPC = 16; __VA_ARGS__; } break; \
-    case 32: { constexpr int LPC = 32; __VA_ARGS__; } break; \
-    default: { constexpr int LPC = 64; __VA_ARGS__; } break; \
+static inline bool use_fast(const sedcol_params &p) { return p.arith == SEDCOL_ARITH_FAST && p.bcdown != 2; }
+
+#define SC_DISPATCH_LPC(lpc, ...)                                \
+  switch (lpc) {                                                 \
+    case 16: { constexpr int LPC = 16; __VA_ARGS__; } break;     \
+    case 32: { constexpr int LPC = 32; __VA_ARGS__; } break;     \
+    default: { constexpr int LPC = 64; __VA_ARGS__; } break;     \
   }
+#define SC_DISPATCH_ARITH(p, ...)                                          \
+  if (use_fast(p)) { constexpr int AR = ARITH_FAST; __VA_ARGS__; }         \
+  else { constexpr int AR = ARITH_REF; __VA_ARGS__; }
 
 void launch_rhs(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                 const double *conc, double *rhs, double *flux)
 {
-  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_rhs<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n, p, rec,
-                                          conc, rhs, flux));
+  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_rhs<LPC, AR>), grid_for<LPC>(n), dim3(BLOCK), 0,
+                                                               st, d, list, n, p, rec, conc, rhs, flux)));
 }
 
 void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *list, int n,
                   const sedcol_params &p, int rec, double dt, int nsteps, int *errflag)
 {
-#define SC_FIX(M) \
-  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_fixed<LPC, M>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n, p, \
-                                          rec, dt, nsteps, errflag))
+#define SC_FIX(M)                                                                                                   \
+  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_fixed<LPC, M, AR>), grid_for<LPC>(n), dim3(BLOCK), \
+                                                               0, st, d, list, n, p, rec, dt, nsteps, errflag)))
   switch (method) {
     case SEDCOL_EULER: SC_FIX(SEDCOL_EULER); break;
     case SEDCOL_RUNGE_KUTTA_4: SC_FIX(SEDCOL_RUNGE_KUTTA_4); break;
@@ -692,18 +743,39 @@ void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *
 #undef SC_FIX
 }
 
-void launch_emp_adaptive(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p,
-                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *errflag)
+// blocks of a work-queue kernel that are resident at once on this device
+template <int LPC, int AR>
+static int emp_resident_blocks()
 {
-  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_emp_adaptive<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n,
-                                          p, t_start, t_end, r0, r1, max_attempts, errflag));
+  static int cached = 0;
+  if (cached == 0) {
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive<LPC, AR>, BLOCK, 0) != hipSuccess)
+      per_cu = 1;
+    cached = (per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount;
+  }
+  return cached;
+}
+
+void launch_emp_adaptive(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p,
+                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *next,
+                         int *errflag)
+{
+  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, {
+    const unsigned need = grid_for<LPC>(n).x;
+    const unsigned cap = (unsigned)emp_resident_blocks<LPC, AR>();
+    hipLaunchKernelGGL((k_emp_adaptive<LPC, AR>), dim3(need < cap ? need : cap), dim3(BLOCK), 0, st, d, list, n, p,
+                       t_start, t_end, r0, r1, max_attempts, next, errflag);
+  }));
 }
 
 void launch_rhs_store(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                       double *r_out, double *s_out)
 {
-  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_rhs_store<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n, p,
-                                          rec, r_out, s_out));
+  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_rhs_store<LPC, AR>), grid_for<LPC>(n), dim3(BLOCK),
+                                                               0, st, d, list, n, p, rec, r_out, s_out)));
 }
 
 void launch_ae_check(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double dt_red, double rcm,
@@ -741,8 +813,8 @@ void launch_inventory(int lpc, hipStream_t st, const Dev &d, const int *list, in
 void launch_residual(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                      const double *cfloor, double *resid)
 {
-  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_residual<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n, p,
-                                          rec, cfloor, resid));
+  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_residual<LPC, AR>), grid_for<LPC>(n), dim3(BLOCK),
+                                                               0, st, d, list, n, p, rec, cfloor, resid)));
 }
 
 }  // namespace sc

@@ -19,6 +19,7 @@ struct Dev {
   int nrec;
   double t0, dt_rec;
   const double *temp, *cbw, *fup;
+  long long fup_rec_stride;  // NV*ncol, or 0 when one flux record serves all
   double *conc;     // NV*ncell
   double *cellacc;  // 5*ncell: DIC, N2, clamp C, clamp N, clamp P (per m3 pore water)
   double *colacc;   // NV*ncol: time-integrated top flux
@@ -30,8 +31,10 @@ void launch_rhs(int lpc, hipStream_t st, const Dev &d, const int *list, int n, c
                 const double *conc, double *rhs, double *flux);
 void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *list, int n,
                   const sedcol_params &p, int rec, double dt, int nsteps, int *errflag);
+// next: a device int set to 0 before the launch (work-queue counter)
 void launch_emp_adaptive(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p,
-                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *errflag);
+                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *next,
+                         int *errflag);
 void launch_rhs_store(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                       double *r_out, double *s_out);
 void launch_ae_check(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double dt_red, double rcm,

@@ -58,13 +58,13 @@ class Params(ctypes.Structure):
 
 class ColDiag(ctypes.Structure):
     _fields_ = [("accepted", ctypes.c_int64), ("rejected", ctypes.c_int64), ("forced", ctypes.c_int64),
-                ("zero_sink", ctypes.c_int64), ("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double),
+                ("zero_sink", ctypes.c_int64), ("newton", ctypes.c_int64), ("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double),
                 ("dt_minacc", ctypes.c_double), ("p_min", ctypes.c_double), ("status", ctypes.c_int32),
                 ("pad", ctypes.c_int32)]
 
 
 DIAG_DTYPE = np.dtype([("accepted", np.int64), ("rejected", np.int64), ("forced", np.int64),
-                       ("zero_sink", np.int64), ("dt_last", np.float64), ("dt_next", np.float64),
+                       ("zero_sink", np.int64), ("newton", np.int64), ("dt_last", np.float64), ("dt_next", np.float64),
                        ("dt_minacc", np.float64), ("p_min", np.float64), ("status", np.int32),
                        ("pad", np.int32)])
 
@@ -92,7 +92,7 @@ def lib() -> ctypes.CDLL:
             "sedcol_set_params": (ctypes.c_int, [vp, ctypes.POINTER(Params)]),
             "sedcol_set_geometry": (ctypes.c_int, [vp, pd, pd, pd, pd, pd, pd]),
             "sedcol_set_forcing": (ctypes.c_int, [vp, pd, pd, pd]),
-            "sedcol_set_forcing_series": (ctypes.c_int, [vp, i32, dbl, dbl, pd, pd, pd]),
+            "sedcol_set_forcing_series": (ctypes.c_int, [vp, i32, dbl, dbl, pd, pd, pd, i32]),
             "sedcol_set_state": (ctypes.c_int, [vp, pd]),
             "sedcol_get_state": (ctypes.c_int, [vp, pd]),
             "sedcol_rhs": (ctypes.c_int, [vp, pd, pd, pd]),
@@ -187,9 +187,11 @@ class SedimentColumns:
         check(lib().sedcol_set_forcing(self._h, _pd(t), _pd(c), _pd(f)))
 
     def set_forcing_series(self, temp, cbw, fup, t0, dt_rec):
+        """temp (nrec, ncol), cbw (nrec, NV, ncol), fup (nrec or 1, NV, ncol)."""
         t, c, f = _c(temp), _c(cbw), _c(fup)
         nrec = int(t.shape[0])
-        check(lib().sedcol_set_forcing_series(self._h, nrec, float(t0), float(dt_rec), _pd(t), _pd(c), _pd(f)))
+        nrf = int(f.shape[0]) if f.ndim == 3 else 1
+        check(lib().sedcol_set_forcing_series(self._h, nrec, float(t0), float(dt_rec), _pd(t), _pd(c), _pd(f), nrf))
 
     def set_state(self, conc):
         x = _c(conc)

@@ -78,9 +78,11 @@ class Domain:
         return bool(np.all(self.nlayers == self.nlayers[0]))
 
     def record(self, r: int):
-        """(temp, cbw, fup) of forcing record r, contiguous."""
+        """(temp, cbw, fup) of forcing record r, contiguous.  fup may hold a
+        single record that applies to the whole series."""
+        rf = r if self.fup.shape[0] > 1 else 0
         return (np.ascontiguousarray(self.temp[r]), np.ascontiguousarray(self.cbw[r]),
-                np.ascontiguousarray(self.fup[r]))
+                np.ascontiguousarray(self.fup[rf]))
 
     def subset(self, cols) -> "Domain":
         """Domain made of the given columns (in that order)."""
@@ -261,7 +263,6 @@ def shelf_sea(n=300, K=30, land_fraction=0.4, sigma=10.0, days=365, seed=0, tota
                                   * (1.0 + sign * seasonal_amp * season + noise * eps), 0.0)
     d.temp = temp
     d.cbw = cbw
-    d.fup = np.repeat(d.fup, days, axis=0)
     d.t0 = 0.0
     d.dt_rec = SECONDS_PER_DAY
     return d

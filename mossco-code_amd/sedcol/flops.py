@@ -1,0 +1,66 @@
+"""Algorithmic FP64 operation counts of the hot path.
+
+Counted once from the formulas of mossco-code_amd/csrc/sedcol_device.hpp and
+sedcol_kernels.hip (add, sub, mul, div = 1; fma = 2; sign flips, min/max,
+comparisons and selects = 0).  Division and the per-column exp count as one
+operation each, so issued instructions exceed these figures (an FP64 divide
+expands to about ten VALU instructions).
+
+Per cell:
+  kinetics()                       66   (detritus 16, pathway split 21,
+                                         nitrification/ODU/P 17, rates 12)
+  transport per particulate var    12   (x=c*phi 1, F 3, VF 2, divergence 3,
+                                         rescale 3, +R 1 ... counted below)
+  transport per dissolved var       7   (F 3, divergence 3, +R 1)
+  rhs = kinetics + 6*12 + 5*7   = 173
+Adaptive EMP attempt (two stages, per cell):
+  2 rhs                           346
+  Patankar set-up, 2 x 11 x 2      44   (h*g, /c)
+  stage c1 = c + (h p1) f0         22
+  g = (f0 + f1) * 0.5              22
+  error norm 11 x 6                66
+  accepted update 11 x 2 + sinks   26
+Newton iteration (per cell):       70   (11 x (fma 2 + 3 mul + 1 add) + 4 for
+                                         one (P, D) tree combine)
+RK4 step (per cell): 4 rhs + stage updates 3 x 11 x 2 + combination
+  11 x (2 + 2) + sinks 12  = 692 + 66 + 44 + 12 = 814.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NV = 11
+N_PART, N_DIS = 6, 5
+KINETICS = 16 + 21 + 17 + 12
+TRANSPORT_PART = 12
+TRANSPORT_DIS = 7
+RHS_PER_CELL = KINETICS + N_PART * TRANSPORT_PART + N_DIS * TRANSPORT_DIS  # 173
+EMP_ATTEMPT_PER_CELL = 2 * RHS_PER_CELL + 2 * NV * 2 + NV * 2 + NV * 2 + NV * 6 + (NV * 2 + 4)
+NEWTON_PER_CELL = NV * 6 + 4
+RK4_STEP_PER_CELL = 4 * RHS_PER_CELL + 3 * NV * 2 + NV * 4 + 12
+
+#: HBM bytes a persistent launch moves per cell and per column, independent
+#: of the number of steps inside it: state in+out (2*11*8), cell
+#: accumulators in+out (2*5*8), geometry (dz, dzc, phi, phiI, bfac: 5*8);
+#: per column: offsets, bioturbation, flux/colacc in+out, diagnostics, and
+#: per forcing record temperature + 2*11 forcing values.
+BYTES_PER_CELL_LAUNCH = 2 * NV * 8 + 2 * 5 * 8 + 5 * 8
+BYTES_PER_COLUMN_LAUNCH = 8 + 8 + 2 * 2 * NV * 8 + 2 * 88
+BYTES_PER_COLUMN_RECORD = 8 + 2 * NV * 8
+
+
+def emp_flops(cells_per_col, attempts, newton):
+    """FLOPs of adaptive-EMP work: attempts and Newton iterations per column."""
+    K = np.asarray(cells_per_col, dtype=np.float64)
+    return float(np.sum(K * (EMP_ATTEMPT_PER_CELL * np.asarray(attempts, dtype=np.float64)
+                             + NEWTON_PER_CELL * np.asarray(newton, dtype=np.float64))))
+
+
+def rk4_flops(cells_per_col, steps):
+    K = np.asarray(cells_per_col, dtype=np.float64)
+    return float(np.sum(K * RK4_STEP_PER_CELL * np.asarray(steps, dtype=np.float64)))
+
+
+def launch_bytes(cells_per_col, records=1):
+    K = np.asarray(cells_per_col, dtype=np.float64)
+    return float(np.sum(K * BYTES_PER_CELL_LAUNCH + BYTES_PER_COLUMN_LAUNCH + records * BYTES_PER_COLUMN_RECORD))

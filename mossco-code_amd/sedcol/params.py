@@ -70,6 +70,7 @@ class SedParams:
     bcup_part: int = 1                 # bcup_particulate_variables
     bcdown: int = 3
     clamp: int = 1
+    arith: int = 0                     # 0 reference operation order, 1 fast form
 
     def si(self) -> dict:
         """Flat dict in the field order of sedcol_params / oc_params."""
@@ -103,6 +104,8 @@ class SedParams:
             "bcup_part": int(self.bcup_part),
             "bcdown": int(self.bcdown),
             "clamp": int(self.clamp),
+            "arith": int(self.arith),
+            "reserved": 0,
         }
 
 
@@ -117,6 +120,7 @@ PARAM_FIELDS = [
     ("emp_dt_min", ctypes.c_double), ("emp_dt_max", ctypes.c_double),
     ("bcup_dis", ctypes.c_int32), ("bcup_part", ctypes.c_int32),
     ("bcdown", ctypes.c_int32), ("clamp", ctypes.c_int32),
+    ("arith", ctypes.c_int32), ("reserved", ctypes.c_int32),
 ]
 
 

@@ -29,6 +29,7 @@ _PARAM_FIELDS = [
     ("emp_dt_min", ctypes.c_double), ("emp_dt_max", ctypes.c_double),
     ("bcup_dis", ctypes.c_int32), ("bcup_part", ctypes.c_int32),
     ("bcdown", ctypes.c_int32), ("clamp", ctypes.c_int32),
+    ("arith", ctypes.c_int32), ("reserved", ctypes.c_int32),
 ]
 
 
@@ -48,13 +49,13 @@ class OcDomain(ctypes.Structure):
 
 class OcColDiag(ctypes.Structure):
     _fields_ = [("accepted", ctypes.c_int64), ("rejected", ctypes.c_int64), ("forced", ctypes.c_int64),
-                ("zero_sink", ctypes.c_int64), ("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double),
+                ("zero_sink", ctypes.c_int64), ("newton", ctypes.c_int64), ("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double),
                 ("dt_minacc", ctypes.c_double), ("p_min", ctypes.c_double), ("status", ctypes.c_int32),
                 ("pad", ctypes.c_int32)]
 
 
 DIAG_DTYPE = np.dtype([("accepted", np.int64), ("rejected", np.int64), ("forced", np.int64),
-                       ("zero_sink", np.int64), ("dt_last", np.float64), ("dt_next", np.float64),
+                       ("zero_sink", np.int64), ("newton", np.int64), ("dt_last", np.float64), ("dt_next", np.float64),
                        ("dt_minacc", np.float64), ("p_min", np.float64), ("status", np.int32),
                        ("pad", np.int32)])
 

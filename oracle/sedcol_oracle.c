@@ -128,8 +128,14 @@ void oc_kinetics(const oc_params *p, double fT, const double *c, double *R,
   double Pprod = (kL * dP) * lim;
 
   double oxlim = O2 / (O2 + p->ksO2oxic);
-  double dnlim = (p->kinO2denit / (O2 + p->kinO2denit)) * (NO3 / (NO3 + p->ksNO3denit));
-  double anlim = (p->kinO2anox / (O2 + p->kinO2anox)) * (p->kinNO3anox / (NO3 + p->kinNO3anox));
+  double dnlim, anlim;
+  if (p->arith == 0) {
+    dnlim = (p->kinO2denit / (O2 + p->kinO2denit)) * (NO3 / (NO3 + p->ksNO3denit));
+    anlim = (p->kinO2anox / (O2 + p->kinO2anox)) * (p->kinNO3anox / (NO3 + p->kinNO3anox));
+  } else { /* fast form: one quotient per limitation term */
+    dnlim = (p->kinO2denit * NO3) / ((O2 + p->kinO2denit) * (NO3 + p->ksNO3denit));
+    anlim = (p->kinO2anox * p->kinNO3anox) / ((O2 + p->kinO2anox) * (NO3 + p->kinNO3anox));
+  }
   double rescale = 1.0 / ((oxlim + dnlim) + anlim);
   double OxicMin = (Cprod * oxlim) * rescale;
   double Denit = (Cprod * dnlim) * rescale;
@@ -186,6 +192,39 @@ static void make_view(const oc_domain *d, int c, colview *v)
   }
 }
 
+/* Fast form of the column transport (p->arith == 1): the same fluxes with the
+ * divisions by dzc (dz at the top) folded into G = D/dzc and the divergence
+ * (F_k - F_k+1)/(VF dz), rescaled by (1-phi)/phi for particulates, written
+ * as (F_k - F_k+1) * W with W = 1/(phi dz) for every variable. */
+static void col_transport_fast(const oc_params *p, const colview *v,
+                               const double *c, long ks, double *rhs, long rs,
+                               double *F1, const double *Dbio, const double *Ddis)
+{
+  const int K = v->K;
+  double Gb[OC_KMAX], Gd[OC_KMAX], W[OC_KMAX], x[OC_KMAX], F[OC_KMAX + 1];
+  for (int k = 0; k < K; ++k) {
+    double len = k == 0 ? v->dz[k] : v->dzc[k];
+    Gb[k] = Dbio[k] / len;
+    Gd[k] = Ddis[k] / len;
+    W[k] = 1.0 / (v->phi[k] * v->dz[k]);
+  }
+  for (int n = 0; n < NV; ++n) {
+    const int part = oc_part[n];
+    const double *G = part ? Gb : Gd;
+    const double *cn = c + (long)n * ks;
+    for (int k = 0; k < K; ++k) x[k] = part ? cn[k] * v->phi[k] : cn[k];
+    for (int k = 1; k < K; ++k) F[k] = -(G[k] * (x[k] - x[k - 1]));
+    int bc = part ? p->bcup_part : p->bcup_dis;
+    if (bc == 1) F[0] = v->fup[n];
+    else if (bc == 2) F[0] = -(G[0] * (x[0] - v->cbw[n]));
+    else F[0] = 0.0;
+    F[K] = 0.0;
+    double *rn = rhs + (long)n * rs;
+    for (int k = 0; k < K; ++k) rn[k] = (F[k] - F[k + 1]) * W[k];
+    F1[n] = F[0];
+  }
+}
+
 /* One get_rhs for one column.  c and rhs are [NV][K] with stride ks.
  * F1[n] = top interface flux (intFlux(:,:,1)); sC/sN per-cell sinks.
  * fabm_sediment_driver.F90:575-717 and diff3d :739-825. */
@@ -201,7 +240,10 @@ static void col_rhs(const oc_params *p, const colview *v, const double *c,
     Dbio[k] = ((((v->B * fT) / 86400.0) / 10000.0) * (1.0 - v->phiI[k])) * v->bfac[k];
     Ddis[k] = Dbio[k] + (((p->D0 + v->T * 0.035) * v->phiI[k]) / 86400.0) / 10000.0;
   }
-  for (int n = 0; n < NV; ++n) {
+  /* the fast form covers BcUp 1..3 and BcDown 1, 3 */
+  const int fast = p->arith == 1 && p->bcdown != 2 && p->bcup_part != 4;
+  if (fast) col_transport_fast(p, v, c, ks, rhs, rs, F1, Dbio, Ddis);
+  for (int n = 0; n < NV && !fast; ++n) {
     const int part = oc_part[n];
     const double *D = part ? Dbio : Ddis;
     const double *cn = c + (long)n * ks;
@@ -293,7 +335,7 @@ static void tree_pd(double *P, double *D, int K)
  * from p = 0 increases monotonically to the root; the product and its
  * derivative are carried together as (P, D) pairs. */
 static double col_psolve(const double *c, long ks, const double *g, long gs,
-                         double dt, int K, int64_t *zero_sink)
+                         double dt, int K, int64_t *zero_sink, int64_t *newton)
 {
   double a[NV][OC_KMAX];
   for (int n = 0; n < NV; ++n)
@@ -309,6 +351,7 @@ static double col_psolve(const double *c, long ks, const double *g, long gs,
   double pp = 0.0;
   double P[OC_KMAX], D[OC_KMAX];
   for (int it = 0; it < 60; ++it) {
+    ++*newton;
     for (int k = 0; k < K; ++k) {
       double Pl = 1.0, Dl = 0.0;
       for (int n = 0; n < NV; ++n) {
@@ -433,7 +476,7 @@ static int col_step(const oc_params *p, const colview *v, int method,
     memcpy(F1out, w->F[3], sizeof(double) * NV);
   } else { /* EMP-1 / EMP-2, fixed step */
     col_rhs(p, v, c, ks, w->r[0], S, w->F[0], w->s[0][0], w->s[0][1]);
-    double p1 = col_psolve(c, ks, w->r[0], S, dt, K, &dg->zero_sink);
+    double p1 = col_psolve(c, ks, w->r[0], S, dt, K, &dg->zero_sink, &dg->newton);
     if (p1 < dg->p_min) dg->p_min = p1;
     if (method == OC_EMP1) {
       for (int n = 0; n < NV; ++n)
@@ -455,7 +498,7 @@ static int col_step(const oc_params *p, const colview *v, int method,
       for (int n = 0; n < NV; ++n)
         for (int k = 0; k < K; ++k)
           w->r[2][n * S + k] = (w->r[0][n * S + k] + w->r[1][n * S + k]) * 0.5;
-      double p2 = col_psolve(c, ks, w->r[2], S, dt, K, &dg->zero_sink);
+      double p2 = col_psolve(c, ks, w->r[2], S, dt, K, &dg->zero_sink, &dg->newton);
       if (p2 < dg->p_min) dg->p_min = p2;
       for (int n = 0; n < NV; ++n)
         for (int k = 0; k < K; ++k) c[n * ks + k] = c[n * ks + k] + (dt * p2) * w->r[2][n * S + k];
@@ -509,7 +552,7 @@ static int col_emp_adaptive(const oc_params *p, const colview *v, double *c,
     int clipped = !(dt < rem);
     double h = clipped ? rem : dt;
     col_rhs(p, v, c, ks, w->r[0], S, w->F[0], w->s[0][0], w->s[0][1]);
-    double p1 = col_psolve(c, ks, w->r[0], S, h, K, &dg->zero_sink);
+    double p1 = col_psolve(c, ks, w->r[0], S, h, K, &dg->zero_sink, &dg->newton);
     for (int n = 0; n < NV; ++n)
       for (int k = 0; k < K; ++k) {
         double y = c[n * ks + k] + (h * p1) * w->r[0][n * S + k];
@@ -519,7 +562,7 @@ static int col_emp_adaptive(const oc_params *p, const colview *v, double *c,
     for (int n = 0; n < NV; ++n)
       for (int k = 0; k < K; ++k)
         g[n * S + k] = (w->r[0][n * S + k] + w->r[1][n * S + k]) * 0.5;
-    double p2 = col_psolve(c, ks, g, S, h, K, &dg->zero_sink);
+    double p2 = col_psolve(c, ks, g, S, h, K, &dg->zero_sink, &dg->newton);
     double err = 0.0;
     int bad = 0;
     for (int n = 0; n < NV; ++n)

@@ -62,6 +62,8 @@ typedef struct {
   int32_t bcup_part;    /* 1 flux, 3 zero, 4 capped distributed flux */
   int32_t bcdown;       /* 1 flux(0), 2 Dirichlet(0), 3 zero-gradient */
   int32_t clamp;        /* clamp to minimum (0) after every step */
+  int32_t arith;        /* 0 reference operation order, 1 fast form */
+  int32_t reserved;
 } oc_params;
 
 typedef struct {
@@ -87,6 +89,7 @@ typedef struct {
   int64_t rejected;   /* rejected attempts */
   int64_t forced;     /* accepted at dt_min with err > 1 */
   int64_t zero_sink;  /* destruction of an exhausted component (EMP) */
+  int64_t newton;     /* Newton iterations of the Patankar factor */
   double dt_last;     /* last accepted step size */
   double dt_next;     /* controller proposal carried to the next call */
   double dt_minacc;   /* smallest accepted step */

@@ -27,8 +27,13 @@ def _perturbed(dom, seed=3, lo=0.5, hi=1.5):
     return x * rng.uniform(lo, hi, size=x.shape)
 
 
-def test_rhs_bitwise(gpu_ready):
+ARITHS = [pytest.param(0, id="reference"), pytest.param(1, id="fast")]
+
+
+@pytest.mark.parametrize("arith", ARITHS)
+def test_rhs_bitwise(gpu_ready, arith):
     dom = cases.plain_columns(seed=0)
+    dom.params.arith = arith
     p = dom.params.si()
     x = _perturbed(dom)
     with capi.SedimentColumns.from_domain(dom) as g:
@@ -52,9 +57,11 @@ def test_rhs_boundary_kinds(gpu_ready, bcup_dis, bcup_part, bcdown):
     assert np.array_equal(fg, fo)
 
 
+@pytest.mark.parametrize("arith", ARITHS)
 @pytest.mark.parametrize("method", [EULER, RUNGE_KUTTA_4, RUNGE_KUTTA_4_38, EMP1, EMP2])
-def test_one_step(gpu_ready, method):
+def test_one_step(gpu_ready, method, arith):
     dom = cases.plain_columns(seed=1)
+    dom.params.arith = arith
     p = dom.params.si()
     x0 = _perturbed(dom, seed=4)
     xg = x0.copy()
@@ -70,9 +77,11 @@ def test_one_step(gpu_ready, method):
     assert np.array_equal(fg, orc.flux)
 
 
-def test_rk4_30_days(gpu_ready):
+@pytest.mark.parametrize("arith", ARITHS)
+def test_rk4_30_days(gpu_ready, arith):
     """Plain 4x4x30 RK4 case, dt = 360 s, 7200 steps in one persistent launch."""
     dom = cases.plain_columns(seed=0)
+    dom.params.arith = arith
     p = dom.params.si()
     with capi.SedimentColumns.from_domain(dom) as g:
         g.run(360.0, 7200, RUNGE_KUTTA_4)
@@ -134,11 +143,13 @@ def test_adaptive_euler_global_rule(gpu_ready):
     assert np.all(acc_g == orc.diag["accepted"])
 
 
-def test_emp_adaptive_shelf_subset(gpu_ready):
+@pytest.mark.parametrize("arith", ARITHS)
+def test_emp_adaptive_shelf_subset(gpu_ready, arith):
     """Adaptive EMP across daily forcing records on shelf-sea columns."""
     full = cases.shelf_sea(n=40, days=3, seed=11)
     wet = np.flatnonzero(full.mask == 0)[:64]
     dom = full.subset(wet)
+    dom.params.arith = arith
     p = dom.params.si()
     t1 = 3 * 86400.0
     with capi.SedimentColumns.from_domain(dom) as g:
@@ -164,9 +175,11 @@ def test_emp_adaptive_shelf_subset(gpu_ready):
         assert r.max() <= 1e-12, (e, r.max())
 
 
-def test_emp_ragged(gpu_ready):
+@pytest.mark.parametrize("arith", ARITHS)
+def test_emp_ragged(gpu_ready, arith):
     dom = cases.ragged_columns(ncol=96, seed=4)
     assert set(np.unique(np.where(dom.nlayers <= 16, 16, np.where(dom.nlayers <= 32, 32, 64)))) == {16, 32, 64}
+    dom.params.arith = arith
     p = dom.params.si()
     with capi.SedimentColumns.from_domain(dom) as g:
         g.integrate(0.0, 86400.0)
