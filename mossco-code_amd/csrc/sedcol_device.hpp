@@ -306,23 +306,41 @@ __device__ __forceinline__ void column_rhs(const sedcol_params &p, const Lane &L
   __builtin_amdgcn_wave_barrier();
 }
 
-// Patankar factor of one column: root p in (0,1] of
-// p = prod_{j: g_j<0} (1 + h g_j p / c_j) (Bruggeman et al. 2007), by Newton
-// from p = 0 on the convex decreasing h(p) = prod - p.
-template <int LPC>
-__device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g)[NV], double h, bool active,
-                                         int64_t &zero_sink, int64_t &newton)
+template <int LPC, int S = 1>
+__device__ __forceinline__ double lg_prod(double v)
+{
+  if constexpr (S >= LPC) return v;
+  else return lg_prod<LPC, 2 * S>(v * bfly<S>(v));
+}
+
+// Patankar factor of one column (Bruggeman et al. 2007), by Newton from
+// p = 0 on the convex decreasing h(p) = S prod (1 + a_j p) - p,
+// a_j = h g_j / c_j over the destroyed components (g_j < 0).
+//   first stage (STAGE2 = false): S = 1, root in (0, 1];
+//   second stage: p = prod (c_j + h g_j p) / c1_j with c1 the first-stage
+//   values (den), i.e. S = prod c_j / c1_j (factors with c1_j <= 0 left
+//   out); the c1 denominators make the two-stage scheme second order (the
+//   weights of MPRK22, Burchard et al. 2003, in single-factor form).
+template <int LPC, bool STAGE2>
+__device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g)[NV], const double (&den)[NV],
+                                         double h, bool active, int64_t &zero_sink, int64_t &newton)
 {
   double a[NV];
+  double Sl = 1.0;
 #pragma unroll
   for (int n = 0; n < NV; ++n) {
     double av = 0.0;
     if (active && g[n] < 0.0) {
-      if (c[n] > 0.0) av = (h * g[n]) / c[n];
-      else zero_sink += 1;
+      if (c[n] > 0.0) {
+        av = (h * g[n]) / c[n];
+        if (STAGE2 && den[n] > 0.0) Sl = Sl * (c[n] / den[n]);
+      } else {
+        zero_sink += 1;
+      }
     }
     a[n] = av;
   }
+  const double Sc = STAGE2 ? lg_prod<LPC>(Sl) : 1.0;
   double pp = 0.0;
   for (int it = 0; it < 60; ++it) {
     newton += 1;
@@ -334,15 +352,15 @@ __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g
       P = P * f;
     }
     lg_pd<LPC>(P, D);
-    double hv = P - pp;
-    double hp = D - 1.0;
+    double hv = Sc * P - pp;
+    double hp = Sc * D - 1.0;
     double pn = pp - hv / hp;
     if (!(pn > pp)) break;
     bool conv = (pn - pp) <= 0x1p-50 * pn;
     pp = pn;
     if (conv) break;
   }
-  return pp > 1.0 ? 1.0 : pp;
+  return (!STAGE2 && pp > 1.0) ? 1.0 : pp;
 }
 
 }  // namespace sc

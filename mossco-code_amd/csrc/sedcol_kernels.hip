@@ -235,7 +235,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
         f0[n] = r;
         if (l == 0) F[0][n] = Fv;
       });
-      const double p1 = psolve<LPC>(x, f0, dt, act, zs, dg.newton);
+      const double p1 = psolve<LPC, false>(x, f0, x, dt, act, zs, dg.newton);
       if (p1 < dg.p_min) dg.p_min = p1;
       if (METHOD == SEDCOL_EMP1) {
 #pragma unroll
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
           f0[n] = (f0[n] + r) * 0.5;
           if (l == 0) F[1][n] = Fv;
         });
-        const double p2 = psolve<LPC>(x, f0, dt, act, zs, dg.newton);
+        const double p2 = psolve<LPC, true>(x, f0, c1, dt, act, zs, dg.newton);
         if (p2 < dg.p_min) dg.p_min = p2;
 #pragma unroll
         for (int n = 0; n < NV; ++n) x[n] = x[n] + (dt * p2) * f0[n];
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(De
         f0[n] = rr;
         if (l == 0) F[0][n] = Fv;
       });
-      const double p1 = psolve<LPC>(x, f0, h, act, zs, dg.newton);
+      const double p1 = psolve<LPC, false>(x, f0, x, h, act, zs, dg.newton);
 #pragma unroll
       for (int n = 0; n < NV; ++n) {
         const double y = x[n] + (h * p1) * f0[n];
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(De
         f0[n] = (f0[n] + rr) * 0.5;
         if (l == 0) F[1][n] = Fv;
       });
-      const double p2 = psolve<LPC>(x, f0, h, act, zs, dg.newton);
+      const double p2 = psolve<LPC, true>(x, f0, c1, h, act, zs, dg.newton);
       double el = 0.0;
       bool bad = false;
 #pragma unroll

@@ -16,6 +16,7 @@ Per cell:
 Adaptive EMP attempt (two stages, per cell):
   2 rhs                           346
   Patankar set-up, 2 x 11 x 2      44   (h*g, /c)
+  second-stage scale, 11 x 2       22   (c/c1, product)
   stage c1 = c + (h p1) f0         22
   g = (f0 + f1) * 0.5              22
   error norm 11 x 6                66
@@ -35,7 +36,7 @@ KINETICS = 16 + 21 + 17 + 12
 TRANSPORT_PART = 12
 TRANSPORT_DIS = 7
 RHS_PER_CELL = KINETICS + N_PART * TRANSPORT_PART + N_DIS * TRANSPORT_DIS  # 173
-EMP_ATTEMPT_PER_CELL = 2 * RHS_PER_CELL + 2 * NV * 2 + NV * 2 + NV * 2 + NV * 6 + (NV * 2 + 4)
+EMP_ATTEMPT_PER_CELL = 2 * RHS_PER_CELL + 2 * NV * 2 + NV * 2 + NV * 2 + NV * 2 + NV * 6 + (NV * 2 + 4)
 NEWTON_PER_CELL = NV * 6 + 4
 RK4_STEP_PER_CELL = 4 * RHS_PER_CELL + 3 * NV * 2 + NV * 4 + 12
 

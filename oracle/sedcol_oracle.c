@@ -313,6 +313,15 @@ static double tree_sum(double *v, int K)
   return v[0];
 }
 
+static double tree_prod(double *v, int K)
+{
+  int q = pow2ceil(K);
+  for (int k = K; k < q; ++k) v[k] = 1.0;
+  for (int s = 1; s < q; s <<= 1)
+    for (int l = 0; l < q; l += 2 * s) v[l] = v[l] * v[l + s];
+  return v[0];
+}
+
 static void tree_pd(double *P, double *D, int K)
 {
   int q = pow2ceil(K);
@@ -334,20 +343,38 @@ static void tree_pd(double *P, double *D, int K)
  * prod - p is convex and decreasing on the admissible interval, so Newton
  * from p = 0 increases monotonically to the root; the product and its
  * derivative are carried together as (P, D) pairs. */
-static double col_psolve(const double *c, long ks, const double *g, long gs,
-                         double dt, int K, int64_t *zero_sink, int64_t *newton)
+/* Second stage (den != NULL): p = prod_{j: g_j<0} (c_j + dt g_j p) / den_j
+ * with den = the first-stage values c1, i.e. p = S prod (1 + a_j p) with the
+ * constant S = prod c_j/den_j (a factor is left out when den_j <= 0).  Using
+ * c1 rather than c in the denominators is what makes the two-stage scheme
+ * second order (the weights c_j^{n+1}/c_j^{(1)} of MPRK22, Burchard et al.
+ * 2003, in the single-factor form of Bruggeman et al. 2007).  The same
+ * Newton iteration applies; S > 1 allows p > 1. */
+static double col_psolve2(const double *c, long ks, const double *g, long gs,
+                          const double *den, long dens, double dt, int K,
+                          int64_t *zero_sink, int64_t *newton)
 {
-  double a[NV][OC_KMAX];
+  double a[NV][OC_KMAX], S[OC_KMAX];
+  for (int k = 0; k < K; ++k) S[k] = 1.0;
   for (int n = 0; n < NV; ++n)
     for (int k = 0; k < K; ++k) {
       double gv = g[(long)n * gs + k], cv = c[(long)n * ks + k];
       double av = 0.0;
       if (gv < 0.0) {
-        if (cv > 0.0) av = (dt * gv) / cv;
-        else ++*zero_sink;
+        if (cv > 0.0) {
+          av = (dt * gv) / cv;
+          if (den) {
+            double dv = den[(long)n * dens + k];
+            if (dv > 0.0) S[k] = S[k] * (cv / dv);
+          }
+        } else {
+          ++*zero_sink;
+        }
       }
       a[n][k] = av;
     }
+  double Sc = 1.0;
+  if (den) Sc = tree_prod(S, K);
   double pp = 0.0;
   double P[OC_KMAX], D[OC_KMAX];
   for (int it = 0; it < 60; ++it) {
@@ -364,16 +391,22 @@ static double col_psolve(const double *c, long ks, const double *g, long gs,
       D[k] = Dl;
     }
     tree_pd(P, D, K);
-    double h = P[0] - pp;
-    double hp = D[0] - 1.0;
+    double h = Sc * P[0] - pp;
+    double hp = Sc * D[0] - 1.0;
     double pn = pp - h / hp;
     if (!(pn > pp)) break;
     int conv = (pn - pp) <= 0x1p-50 * pn;
     pp = pn;
     if (conv) break;
   }
-  if (pp > 1.0) pp = 1.0;
+  if (!den && pp > 1.0) pp = 1.0;
   return pp;
+}
+
+static double col_psolve(const double *c, long ks, const double *g, long gs,
+                         double dt, int K, int64_t *zero_sink, int64_t *newton)
+{
+  return col_psolve2(c, ks, g, gs, NULL, 0, dt, K, zero_sink, newton);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -498,7 +531,7 @@ static int col_step(const oc_params *p, const colview *v, int method,
       for (int n = 0; n < NV; ++n)
         for (int k = 0; k < K; ++k)
           w->r[2][n * S + k] = (w->r[0][n * S + k] + w->r[1][n * S + k]) * 0.5;
-      double p2 = col_psolve(c, ks, w->r[2], S, dt, K, &dg->zero_sink, &dg->newton);
+      double p2 = col_psolve2(c, ks, w->r[2], S, c1, S, dt, K, &dg->zero_sink, &dg->newton);
       if (p2 < dg->p_min) dg->p_min = p2;
       for (int n = 0; n < NV; ++n)
         for (int k = 0; k < K; ++k) c[n * ks + k] = c[n * ks + k] + (dt * p2) * w->r[2][n * S + k];
@@ -562,7 +595,7 @@ static int col_emp_adaptive(const oc_params *p, const colview *v, double *c,
     for (int n = 0; n < NV; ++n)
       for (int k = 0; k < K; ++k)
         g[n * S + k] = (w->r[0][n * S + k] + w->r[1][n * S + k]) * 0.5;
-    double p2 = col_psolve(c, ks, g, S, h, K, &dg->zero_sink, &dg->newton);
+    double p2 = col_psolve2(c, ks, g, S, c1, S, h, K, &dg->zero_sink, &dg->newton);
     double err = 0.0;
     int bad = 0;
     for (int n = 0; n < NV; ++n)

@@ -1,0 +1,46 @@
+"""CPU checks of the C ABI library: it loads and exports every function that
+include/sedcol.h declares, and the host structs match the header layout.
+No compute calls (there is no GPU here)."""
+import ctypes
+import re
+from pathlib import Path
+
+from sedcol import capi
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "sedcol.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sedcol_[a-z_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(capi.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = capi.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_sizes():
+    assert ctypes.sizeof(capi.Params) == 21 * 8 + 6 * 4
+    assert ctypes.sizeof(capi.ColDiag) == 9 * 8 + 2 * 4
+    assert ctypes.sizeof(capi.Desc) == 4 * 4 + 2 * 8
+
+
+def test_create_without_device_fails_cleanly():
+    import numpy as np
+    import torch
+
+    if torch.cuda.is_available():
+        return
+    try:
+        capi.SedimentColumns(np.full(2, 10))
+    except capi.SedcolError as e:
+        assert e.code == capi.SEDCOL_EHIP
+    else:
+        raise AssertionError("handle creation succeeded without a device")

@@ -161,7 +161,7 @@ def test_emp_adaptive_shelf_subset(gpu_ready, arith):
     orc = O.Oracle(dom, p)
     xo = dom.conc0.copy()
     spans = cases.record_spans(0.0, t1, dom.nrec, dom.t0, dom.dt_rec)
-    assert orc.integrate_series(xo, 0.0, t1, spans) == 0
+    assert orc.integrate_series(xo, 0.0, t1, spans, nthreads=8) == 0
     print("EMP accepted", dg["accepted"].sum(), "rejected", dg["rejected"].sum(),
           "max rel", max_rel(xg, xo), flush=True)
     assert np.array_equal(xg, xo)
@@ -181,12 +181,13 @@ def test_emp_ragged(gpu_ready, arith):
     assert set(np.unique(np.where(dom.nlayers <= 16, 16, np.where(dom.nlayers <= 32, 32, 64)))) == {16, 32, 64}
     dom.params.arith = arith
     p = dom.params.si()
+    span = 6 * 3600.0
     with capi.SedimentColumns.from_domain(dom) as g:
-        g.integrate(0.0, 86400.0)
+        g.integrate(0.0, span)
         xg = g.get_state()
     orc = O.Oracle(dom, p)
     xo = dom.conc0.copy()
-    assert orc.emp_integrate(xo, 86400.0) == 0
+    assert orc.emp_integrate(xo, span, nthreads=8) == 0
     assert np.array_equal(xg, xo)
     assert np.all(xg >= 0.0)
 

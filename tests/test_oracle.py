@@ -1,0 +1,337 @@
+"""CPU tests that pin the oracle (oracle/sedcol_oracle.c).
+
+What pins what:
+  * steppers (solver_library.F90:80-189) -- the reference's own
+    solver_library.F90 compiled into oracle/_ref (when built here) and the
+    golden vectors it wrote (tests/golden/ref_steppers.npz), bitwise; closed
+    forms for linear decay (RK4 and RK4-3/8 reproduce the degree-4 Taylor
+    polynomial, Euler 1+z);
+  * ADAPTIVE_EULER's global rule -- reference build / golden vectors, and the
+    no-reject special case equal to EULER;
+  * transport (get_rhs + diff3d) -- telescoping sum phi dz Q = F_top - F_bottom,
+    the uniform Dirichlet steady state, zero divergence of a linear profile,
+    the reference's layer-height / porosity formulas;
+  * kinetics (library definition; FABM hzg_omexdia_p is not available, parity
+    with it is unpinned) -- C/N/P stoichiometry, oxic/anoxic limits, zero
+    destruction of an exhausted species;
+  * EMP -- the closed form c/(1 + k dt) of one linear sink, positivity at any
+    step, conservation, observed orders 1 and 2;
+  * exp/rsqrt -- against libm.
+A planted mistake in any of these formulas fails at least one test here.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from sedcol import cases, grid
+from sedcol.params import EULER, RUNGE_KUTTA_4, ADAPTIVE_EULER, RUNGE_KUTTA_4_38, EMP1, EMP2
+import oracle as O
+from helpers import budget_residuals
+
+GOLDEN = __import__("pathlib").Path(__file__).with_name("golden") / "ref_steppers.npz"
+
+
+# ---------------------------------------------------------------------------
+# reference steppers
+# ---------------------------------------------------------------------------
+
+def _golden_domain():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", GOLDEN.with_name("make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    return mg
+
+
+@pytest.mark.parametrize("name,method,dt,nsteps", [("euler", 0, 60.0, 20), ("rk4", 1, 360.0, 40),
+                                                   ("adaptive_euler", 2, 3600.0, 4), ("rk4_38", 3, 360.0, 40)])
+def test_oracle_matches_reference_golden(name, method, dt, nsteps):
+    mg = _golden_domain()
+    z = np.load(GOLDEN)
+    d = mg.domain()
+    orc = O.Oracle(d, d.params.si())
+    x = z["x0"].copy()
+    for _ in range(nsteps):
+        assert orc.ode_solver(x, dt, method) == 0
+    assert np.array_equal(x, z[name]), f"{name}: max abs diff {np.max(np.abs(x - z[name]))}"
+    if method == ADAPTIVE_EULER:
+        assert orc.last_min_dt[0] == z[name + "_last_min_dt"][0]
+        assert orc.last_min_dt[0] < dt
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (needs the reference tree)")
+@pytest.mark.parametrize("method,dt,nsteps", [(EULER, 100.0, 15), (RUNGE_KUTTA_4, 720.0, 25),
+                                              (ADAPTIVE_EULER, 7200.0, 3), (RUNGE_KUTTA_4_38, 720.0, 25)])
+def test_oracle_matches_reference_build(method, dt, nsteps):
+    d = cases.plain_columns(ni=3, nj=1, K=9, seed=8, total=0.08)
+    d.params.dt_min = 10.0
+    d.params.clamp = 0
+    p = d.params.si()
+    rng = np.random.default_rng(5)
+    x0 = d.conc0.copy()
+    x0[0:2] = rng.uniform(0.0, 3.0e4, size=x0[0:2].shape)
+    x0[8] = rng.uniform(0.0, 50.0, size=x0[8].shape)
+    a = x0.copy()
+    ref = O.Oracle(d, p)
+    ref.ref_ode_solver(a, dt, method, nsteps=nsteps)
+    b = x0.copy()
+    orc = O.Oracle(d, p)
+    for _ in range(nsteps):
+        orc.ode_solver(b, dt, method)
+    assert np.array_equal(a, b)
+    if method == ADAPTIVE_EULER:
+        assert ref.last_min_dt[0] == orc.last_min_dt[0]
+
+
+def _decay_domain(K=4, ncol=2):
+    """Columns where only labile detritus is present and nothing moves:
+    bioturbation 0, no deposition, CprodMax so large that the production cap
+    is exactly 1, so dC/dt = -kL C exactly."""
+    d = cases.plain_columns(ni=ncol, nj=1, K=K, seed=3, total=0.02)
+    d.biot[:] = 0.0
+    d.fup[:] = 0.0
+    d.cbw[:] = 0.0
+    d.conc0[:] = 0.0
+    d.conc0[0] = np.linspace(100.0, 500.0, d.ncell)
+    d.params.kin.CprodMax = 1.0e300
+    d.params.clamp = 0
+    return d
+
+
+@pytest.mark.parametrize("method", [EULER, RUNGE_KUTTA_4, RUNGE_KUTTA_4_38])
+def test_linear_decay_closed_form(method):
+    d = _decay_domain()
+    p = d.params.si()
+    dt = 6.0 * 3600.0
+    T = d.temp[0]
+    fT = np.array([O.oc_exp(-4500.0 * (1.0 / (t + 273.0) - 1.0 / 288.0)) for t in T])
+    k = fT * p["rL"]
+    x = d.conc0.copy()
+    orc = O.Oracle(d, p)
+    n = 10
+    for _ in range(n):
+        orc.ode_solver(x, dt, method)
+    z = -np.repeat(k, d.nlayers) * dt
+    g = 1.0 + z if method == EULER else 1.0 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    expect = d.conc0[0] * g ** n
+    assert np.allclose(x[0], expect, rtol=1e-13, atol=0)
+    # the other particulate pools stay empty (the dissolved products diffuse
+    # with explicit steps far above their stability limit and are not checked)
+    assert np.all(x[1:5] == 0.0)
+
+
+def test_adaptive_euler_without_rejects_is_euler():
+    d = _decay_domain()
+    p = d.params.si()
+    a = d.conc0.copy()
+    b = d.conc0.copy()
+    O.Oracle(d, p).ode_solver(a, 600.0, ADAPTIVE_EULER)
+    O.Oracle(d, p).ode_solver(b, 600.0, EULER)
+    assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------------------
+# transport
+# ---------------------------------------------------------------------------
+
+def _transport_only(d):
+    """Zero every reaction: no detritus reactivity, no nitrification, ..."""
+    k = d.params.kin
+    k.rLabile = k.rSemilabile = k.rnit = k.rODUox = k.PAds = 0.0
+    return d
+
+
+@pytest.mark.parametrize("arith", [0, 1])
+@pytest.mark.parametrize("bcup_dis,bcup_part", [(2, 1), (1, 1), (3, 3)])
+def test_transport_telescopes(arith, bcup_dis, bcup_part):
+    d = _transport_only(cases.plain_columns(ni=3, nj=2, K=13, seed=2))
+    d.params.arith = arith
+    d.params.bcup_dis, d.params.bcup_part = bcup_dis, bcup_part
+    p = d.params.si()
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0.0, 100.0, size=d.conc0.shape)
+    r, f = O.Oracle(d, p).rhs(x)
+    w = d.phi * d.dz
+    off = d.off
+    for c in range(d.ncol):
+        sl = slice(off[c], off[c + 1])
+        for n in range(11):
+            s = np.sum(w[sl] * r[n, sl])
+            scale = np.sum(np.abs(w[sl] * r[n, sl])) + abs(f[n, c]) + 1e-300
+            assert abs(s - f[n, c]) <= 1e-13 * scale, (c, n, s, f[n, c])
+    if bcup_dis == 3:
+        assert np.all(f[5:10] == 0.0)
+
+
+def test_dirichlet_uniform_profile_is_steady():
+    d = _transport_only(cases.plain_columns(ni=2, nj=2, K=10, seed=4))
+    d.fup[:] = 0.0
+    no3 = d.cbw[:, 6].copy()
+    d.cbw[:] = 0.0
+    d.cbw[:, 6] = no3
+    p = d.params.si()
+    x = np.zeros_like(d.conc0)
+    colid = np.repeat(np.arange(d.ncol), d.nlayers)
+    x[6] = d.cbw[0, 6][colid]  # nitrate equal to bottom water everywhere
+    r, f = O.Oracle(d, p).rhs(x)
+    assert np.all(r == 0.0)
+    assert np.all(f[6] == 0.0)
+
+
+def test_linear_profile_zero_interior_divergence():
+    d = _transport_only(cases.plain_columns(ni=1, nj=1, K=12, seed=4))
+    K = 12
+    d.dz[:] = 0.01
+    d.dzc[1:] = 0.01
+    d.phi[:] = 0.8
+    d.phiI[:] = 0.8
+    d.bfac[:] = 1.0
+    p = d.params.si()
+    x = np.zeros_like(d.conc0)
+    x[6] = 5.0 + 2.0 * np.arange(K)
+    r, _ = O.Oracle(d, p).rhs(x)
+    assert np.allclose(r[6, 1:K - 1], 0.0, atol=1e-20)
+    assert r[6, K - 1] < 0.0  # closed bottom: the deepest layer loses to the one above
+
+
+def test_grid_formulas():
+    g = grid.init_grid_linear(30, 0.001)
+    assert math.isclose(g["dz"].sum(), 0.18 * 30 / 31, rel_tol=1e-13)
+    assert math.isclose(g["dz"][0], 0.001, rel_tol=1e-15)
+    g = grid.init_grid_geometric(30, 1.0e-3, 0.2)
+    assert math.isclose(g["dz"].sum(), 0.2, rel_tol=1e-12)
+    assert math.isclose(g["dz"][1] / g["dz"][0], 1.110256, rel_tol=1e-6)
+    assert np.allclose(g["dzc"][1:], 0.5 * (g["dz"][1:] + g["dz"][:-1]), rtol=1e-13)
+    phi = grid.porosity_linear(g["zc"])
+    pi = grid.interface_porosity(phi)
+    assert pi[0] == phi[0] and np.allclose(pi[1:], 0.5 * (phi[1:] + phi[:-1]))
+    b = grid.bioturbation_factor(g["zi"][:-1], 1, bioturbation=0.9, depth_cm=5.0, min_cm2d=0.2)
+    assert b[0] == 1.0 and np.isclose(b.min(), 0.2 / 0.9)
+
+
+# ---------------------------------------------------------------------------
+# kinetics (library definition)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("arith", [0, 1])
+def test_kinetics_stoichiometry(arith):
+    p = dict(cases.SedParams(arith=arith).si())
+    rng = np.random.default_rng(9)
+    for _ in range(200):
+        c = rng.uniform(0.0, 1.0, 11) * np.array([5e4, 5e4, 8e3, 8e3, 500, 300, 40, 15, 200, 5, 20])
+        fT = rng.uniform(0.5, 1.5)
+        R, sC, sN = O.kinetics(p, fT, c)
+        scale = np.abs(R).max() + 1e-300
+        assert abs(R[0] + R[1] + sC) <= 4e-16 * scale * 4
+        assert abs(R[2] + R[3] + R[6] + R[7] + sN) <= 1e-15 * scale * 8
+        assert abs(R[4] + R[9] + R[10]) <= 1e-15 * scale * 8
+
+
+def test_kinetics_limits():
+    p = dict(cases.SedParams().si())
+    c = np.array([1e4, 0, 0, 0, 0, 1e9, 0, 0, 0, 0, 0], dtype=float)
+    R, sC, sN = O.kinetics(p, 1.0, c)
+    assert math.isclose(-R[5], sC, rel_tol=1e-6)  # pure oxic mineralisation: O2 use = C production
+    c = np.array([1e4, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0], dtype=float)
+    R, sC, sN = O.kinetics(p, 1.0, c)
+    assert R[5] == 0.0 and R[6] == 0.0 and math.isclose(R[8], sC, rel_tol=1e-15)  # all anoxic -> ODU
+    assert sN == 0.0
+
+
+def test_kinetics_exhausted_species_not_destroyed():
+    p = dict(cases.SedParams().si())
+    rng = np.random.default_rng(4)
+    for j in range(11):
+        c = rng.uniform(1.0, 100.0, 11)
+        c[j] = 0.0
+        R, _, _ = O.kinetics(p, 1.1, c)
+        assert R[j] >= 0.0, j
+
+
+# ---------------------------------------------------------------------------
+# EMP
+# ---------------------------------------------------------------------------
+
+def test_emp1_single_linear_sink_closed_form():
+    d = _decay_domain(K=1, ncol=1)
+    p = d.params.si()
+    k = O.oc_exp(-4500.0 * (1.0 / (d.temp[0, 0] + 273.0) - 1.0 / 288.0)) * p["rL"]
+    for dt in (10.0, 1.0e5, 1.0e9):
+        x = d.conc0.copy()
+        O.Oracle(d, p).ode_solver(x, dt, EMP1)
+        # c + dt p f cancels down to c/(1 + k dt): relative rounding grows
+        # with 1 + k dt
+        assert math.isclose(x[0, 0], d.conc0[0, 0] / (1.0 + k * dt), rel_tol=4e-16 * (1.0 + k * dt))
+
+
+@pytest.mark.parametrize("method", [EMP1, EMP2])
+def test_emp_positive_and_conservative_at_huge_steps(method):
+    d = cases.plain_columns(ni=2, nj=2, K=15, seed=12)
+    d.params.clamp = 0
+    p = d.params.si()
+    x = d.conc0.copy()
+    orc = O.Oracle(d, p)
+    inv0 = orc.inventory(x)
+    for dt in (3600.0, 86400.0, 30 * 86400.0):
+        assert orc.ode_solver(x, dt, method) == 0
+        assert np.all(x >= 0.0)
+    res = budget_residuals(inv0, orc.inventory(x), orc.budget())
+    for e, r in res.items():
+        assert r.max() <= 1e-13, (e, r.max())
+
+
+def test_emp_observed_order():
+    """Two linear decays in a closed one-layer column (no transport at all):
+    EMP1 error ~ dt, EMP2 ~ dt^2."""
+    d = _decay_domain(K=1, ncol=1)
+    d.conc0[1] = d.conc0[0] * 0.5
+    d.params.bcup_dis = 3
+    p = d.params.si()
+    t_end = 40 * 86400.0
+    fT = O.oc_exp(-4500.0 * (1.0 / (d.temp[0, 0] + 273.0) - 1.0 / 288.0))
+    exact = d.conc0[0] * math.exp(-fT * p["rL"] * t_end)
+    orders = {}
+    for method in (EMP1, EMP2):
+        errs = []
+        for n in (20, 40, 80):
+            x = d.conc0.copy()
+            orc = O.Oracle(d, p)
+            for _ in range(n):
+                orc.ode_solver(x, t_end / n, method)
+            errs.append(np.max(np.abs(x[0] - exact) / exact))
+        orders[method] = math.log2(errs[1] / errs[2])
+    assert 0.85 < orders[EMP1] < 1.15, orders
+    assert 1.8 < orders[EMP2] < 2.2, orders
+
+
+def test_emp_adaptive_budget_and_positivity():
+    full = cases.shelf_sea(n=30, days=2, seed=5)
+    dom = full.subset(np.flatnonzero(full.mask == 0)[:24])
+    p = dom.params.si()
+    orc = O.Oracle(dom, p)
+    x = dom.conc0.copy()
+    spans = cases.record_spans(0.0, 2 * 86400.0, dom.nrec, dom.t0, dom.dt_rec)
+    assert orc.integrate_series(x, 0.0, 2 * 86400.0, spans, nthreads=4) == 0
+    assert np.all(x >= 0.0)
+    assert np.all(orc.diag["accepted"] > 48)
+    res = budget_residuals(orc.inventory(dom.conc0), orc.inventory(x), orc.budget())
+    for e, r in res.items():
+        assert r.max() <= 1e-12, (e, r.max())
+
+
+# ---------------------------------------------------------------------------
+# elementary functions
+# ---------------------------------------------------------------------------
+
+def test_exp_within_two_ulp():
+    xs = np.concatenate([np.linspace(-700, 700, 2001), np.linspace(-1, 1, 2001), [-0.62, 0.27, 0.0]])
+    for x in xs:
+        e = O.oc_exp(float(x))
+        ref = math.exp(float(x))
+        assert abs(e - ref) <= 2 * math.ulp(ref), (x, e, ref)
+
+
+def test_rsqrt():
+    for x in np.geomspace(1e-30, 1e30, 301):
+        assert math.isclose(O.oc_rsqrt(float(x)), 1.0 / math.sqrt(float(x)), rel_tol=1e-15)
