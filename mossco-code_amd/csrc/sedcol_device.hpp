@@ -117,26 +117,44 @@ __device__ __forceinline__ void kinetics(const sedcol_params &p, double fT, cons
   double NpS = (kS * sdN) * lim;
   double Pprod = (kL * dP) * lim;
 
-  double oxlim = O2 / (O2 + p.ksO2oxic);
-  double dnlim, anlim;
+  double oxlim, dnlim, anlim, nitlim, o2odu, odulim;
   if (ARITH == ARITH_REF) {
+    oxlim = O2 / (O2 + p.ksO2oxic);
     dnlim = (p.kinO2denit / (O2 + p.kinO2denit)) * (NO3 / (NO3 + p.ksNO3denit));
     anlim = (p.kinO2anox / (O2 + p.kinO2anox)) * (p.kinNO3anox / (NO3 + p.kinNO3anox));
+    nitlim = O2 / (O2 + p.ksO2nitri);
+    o2odu = O2 / (O2 + p.ksO2oduox);
+    odulim = ODU / (ODU + p.PAdsODU);
   } else {
-    dnlim = (p.kinO2denit * NO3) / ((O2 + p.kinO2denit) * (NO3 + p.ksNO3denit));
-    anlim = (p.kinO2anox * p.kinNO3anox) / ((O2 + p.kinO2anox) * (NO3 + p.kinNO3anox));
+    // the six Monod denominators share one reciprocal of their product,
+    // split back with prefix/suffix products (range: concentrations < 1e30,
+    // half-saturation constants >= 1e-6)
+    const double q2 = O2 + p.ksO2oxic;
+    const double q3 = (O2 + p.kinO2denit) * (NO3 + p.ksNO3denit);
+    const double q4 = (O2 + p.kinO2anox) * (NO3 + p.kinNO3anox);
+    const double q6 = O2 + p.ksO2nitri;
+    const double q7 = O2 + p.ksO2oduox;
+    const double q8 = ODU + p.PAdsODU;
+    const double a23 = q2 * q3, a234 = a23 * q4, a2346 = a234 * q6, a23467 = a2346 * q7;
+    const double inv = 1.0 / (a23467 * q8);
+    const double b87 = q8 * q7, b876 = b87 * q6, b8764 = b876 * q4, b87643 = b8764 * q3;
+    oxlim = O2 * (inv * b87643);
+    dnlim = (p.kinO2denit * NO3) * ((inv * q2) * b8764);
+    anlim = (p.kinO2anox * p.kinNO3anox) * ((inv * a23) * b876);
+    nitlim = O2 * ((inv * a234) * b87);
+    o2odu = O2 * ((inv * a2346) * q8);
+    odulim = ODU * (inv * a23467);
   }
   double rescale = 1.0 / ((oxlim + dnlim) + anlim);
   double OxicMin = (Cprod * oxlim) * rescale;
   double Denit = (Cprod * dnlim) * rescale;
   double AnoxMin = (Cprod * anlim) * rescale;
 
-  double Nitri = ((fT * p.rnit) * NH4) * (O2 / (O2 + p.ksO2nitri));
-  double o2odu = O2 / (O2 + p.ksO2oduox);
+  double Nitri = ((fT * p.rnit) * NH4) * nitlim;
   double OduOx = ((fT * p.rODUox) * ODU) * o2odu;
   double kP = fT * p.rPads;
   double Pads = (kP * PO4) * o2odu;
-  double Pdes = (kP * aP) * (ODU / (ODU + p.PAdsODU));
+  double Pdes = (kP * aP) * odulim;
 
   R[0] = -CpL;
   R[1] = -CpS;
@@ -341,6 +359,8 @@ __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g
     a[n] = av;
   }
   const double Sc = STAGE2 ? lg_prod<LPC>(Sl) : 1.0;
+  // quadratic convergence from below: a correction under 2^-26 p leaves the
+  // next one at rounding level, so the iteration stops there
   double pp = 0.0;
   for (int it = 0; it < 60; ++it) {
     newton += 1;
@@ -348,7 +368,7 @@ __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
       double f = fma(a[n], pp, 1.0);
-      D = D * f + P * a[n];
+      D = fma(D, f, P * a[n]);
       P = P * f;
     }
     lg_pd<LPC>(P, D);
@@ -356,7 +376,7 @@ __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g
     double hp = Sc * D - 1.0;
     double pn = pp - hv / hp;
     if (!(pn > pp)) break;
-    bool conv = (pn - pp) <= 0x1p-50 * pn;
+    bool conv = (pn - pp) <= 0x1p-26 * pn;
     pp = pn;
     if (conv) break;
   }

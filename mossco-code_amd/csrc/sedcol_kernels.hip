@@ -426,18 +426,28 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(De
         if (l == 0) F[1][n] = Fv;
       });
       const double p2 = psolve<LPC, true>(x, f0, c1, h, act, zs, dg.newton);
-      double el = 0.0;
+      // err = max |c2 - c1| / sc: the lane's largest ratio by
+      // cross-multiplication over the variables, one division, then the
+      // maximum over the layers
+      double bn = 0.0, bd = 1.0;
       bool bad = false;
 #pragma unroll
       for (int n = 0; n < NV; ++n) {
         const double y = x[n] + (h * p2) * f0[n];
         const double yp = y < 0.0 ? 0.0 : y;
         const double sc = p.emp_atol + p.emp_rtol * (yp > x[n] ? yp : x[n]);
-        const double e = fabs(yp - c1[n]) / sc;
-        if (act) {
-          bad |= e != e;
-          el = e > el ? e : el;
+        const double dv = fabs(yp - c1[n]);
+        bad |= (dv != dv) || (sc != sc);
+        if (dv * bd > bn * sc) {
+          bn = dv;
+          bd = sc;
         }
+      }
+      double el = bn / bd;
+      bad |= el != el;
+      if (!act) {
+        el = 0.0;
+        bad = false;
       }
       const double err = lg_max<LPC>(el);
       if (group_any<LPC>(bad)) {

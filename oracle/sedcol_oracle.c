@@ -127,26 +127,45 @@ void oc_kinetics(const oc_params *p, double fT, const double *c, double *R,
   double NpS = (kS * sdN) * lim;
   double Pprod = (kL * dP) * lim;
 
-  double oxlim = O2 / (O2 + p->ksO2oxic);
-  double dnlim, anlim;
+  double oxlim, dnlim, anlim, nitlim, o2odu, odulim;
   if (p->arith == 0) {
+    oxlim = O2 / (O2 + p->ksO2oxic);
     dnlim = (p->kinO2denit / (O2 + p->kinO2denit)) * (NO3 / (NO3 + p->ksNO3denit));
     anlim = (p->kinO2anox / (O2 + p->kinO2anox)) * (p->kinNO3anox / (NO3 + p->kinNO3anox));
-  } else { /* fast form: one quotient per limitation term */
-    dnlim = (p->kinO2denit * NO3) / ((O2 + p->kinO2denit) * (NO3 + p->ksNO3denit));
-    anlim = (p->kinO2anox * p->kinNO3anox) / ((O2 + p->kinO2anox) * (NO3 + p->kinNO3anox));
+    nitlim = O2 / (O2 + p->ksO2nitri);
+    o2odu = O2 / (O2 + p->ksO2oduox);
+    odulim = ODU / (ODU + p->PAdsODU);
+  } else {
+    /* fast form: the six Monod denominators share one reciprocal, taken of
+     * their product and split back with prefix/suffix products (valid while
+     * the product stays in range: concentrations < 1e30, half-saturation
+     * constants >= 1e-6) */
+    double q2 = O2 + p->ksO2oxic;
+    double q3 = (O2 + p->kinO2denit) * (NO3 + p->ksNO3denit);
+    double q4 = (O2 + p->kinO2anox) * (NO3 + p->kinNO3anox);
+    double q6 = O2 + p->ksO2nitri;
+    double q7 = O2 + p->ksO2oduox;
+    double q8 = ODU + p->PAdsODU;
+    double a23 = q2 * q3, a234 = a23 * q4, a2346 = a234 * q6, a23467 = a2346 * q7;
+    double inv = 1.0 / (a23467 * q8);
+    double b87 = q8 * q7, b876 = b87 * q6, b8764 = b876 * q4, b87643 = b8764 * q3;
+    oxlim = O2 * (inv * b87643);
+    dnlim = (p->kinO2denit * NO3) * ((inv * q2) * b8764);
+    anlim = (p->kinO2anox * p->kinNO3anox) * ((inv * a23) * b876);
+    nitlim = O2 * ((inv * a234) * b87);
+    o2odu = O2 * ((inv * a2346) * q8);
+    odulim = ODU * (inv * a23467);
   }
   double rescale = 1.0 / ((oxlim + dnlim) + anlim);
   double OxicMin = (Cprod * oxlim) * rescale;
   double Denit = (Cprod * dnlim) * rescale;
   double AnoxMin = (Cprod * anlim) * rescale;
 
-  double Nitri = ((fT * p->rnit) * NH4) * (O2 / (O2 + p->ksO2nitri));
-  double o2odu = O2 / (O2 + p->ksO2oduox);
+  double Nitri = ((fT * p->rnit) * NH4) * nitlim;
   double OduOx = ((fT * p->rODUox) * ODU) * o2odu;
   double kP = fT * p->rPads;
   double Pads = (kP * PO4) * o2odu;
-  double Pdes = (kP * aP) * (ODU / (ODU + p->PAdsODU));
+  double Pdes = (kP * aP) * odulim;
 
   R[OC_LDETC] = -CpL;
   R[OC_SDETC] = -CpS;
@@ -377,6 +396,10 @@ static double col_psolve2(const double *c, long ks, const double *g, long gs,
   if (den) Sc = tree_prod(S, K);
   double pp = 0.0;
   double P[OC_KMAX], D[OC_KMAX];
+  /* Newton converges quadratically from below: once a correction is below
+   * 2^-26 p the next one would be at rounding level, so the iteration stops
+   * there (p below the root keeps every component positive, and the update
+   * c + dt p f conserves for any p). */
   for (int it = 0; it < 60; ++it) {
     ++*newton;
     for (int k = 0; k < K; ++k) {
@@ -384,7 +407,7 @@ static double col_psolve2(const double *c, long ks, const double *g, long gs,
       for (int n = 0; n < NV; ++n) {
         double av = a[n][k];
         double f = fma(av, pp, 1.0);
-        Dl = Dl * f + Pl * av;
+        Dl = fma(Dl, f, Pl * av);
         Pl = Pl * f;
       }
       P[k] = Pl;
@@ -395,7 +418,7 @@ static double col_psolve2(const double *c, long ks, const double *g, long gs,
     double hp = Sc * D[0] - 1.0;
     double pn = pp - h / hp;
     if (!(pn > pp)) break;
-    int conv = (pn - pp) <= 0x1p-50 * pn;
+    int conv = (pn - pp) <= 0x1p-26 * pn;
     pp = pn;
     if (conv) break;
   }
@@ -596,19 +619,27 @@ static int col_emp_adaptive(const oc_params *p, const colview *v, double *c,
       for (int k = 0; k < K; ++k)
         g[n * S + k] = (w->r[0][n * S + k] + w->r[1][n * S + k]) * 0.5;
     double p2 = col_psolve2(c, ks, g, S, c1, S, h, K, &dg->zero_sink, &dg->newton);
+    /* err = max_j |c2_j - c1_j| / sc_j: per layer the largest ratio is found
+     * by cross-multiplication over the variables (in order) and divided
+     * once, then the maximum is taken over the layers */
     double err = 0.0;
     int bad = 0;
-    for (int n = 0; n < NV; ++n)
-      for (int k = 0; k < K; ++k) {
+    for (int k = 0; k < K; ++k) {
+      double bn = 0.0, bd = 1.0;
+      for (int n = 0; n < NV; ++n) {
         double y = c[n * ks + k] + (h * p2) * g[n * S + k];
         cn[n * S + k] = y;
         double yp = y < 0.0 ? 0.0 : y;
         double c0 = c[n * ks + k];
         double sc = p->emp_atol + p->emp_rtol * (yp > c0 ? yp : c0);
-        double e = fabs(yp - c1[n * S + k]) / sc;
-        if (e != e) bad = 1;
-        if (e > err) err = e;
+        double dv = fabs(yp - c1[n * S + k]);
+        if (dv != dv || sc != sc) bad = 1;
+        if (dv * bd > bn * sc) { bn = dv; bd = sc; }
       }
+      double e = bn / bd;
+      if (e != e) bad = 1;
+      if (e > err) err = e;
+    }
     if (bad) { dg->status = 1; break; }
     int accept = (err <= 1.0) || !(h > p->emp_dt_min);
     double fac = err > 0.0 ? 0.9 * oc_rsqrt(err) : 5.0;
