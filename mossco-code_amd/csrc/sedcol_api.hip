@@ -58,6 +58,8 @@ struct sedcol_handle {
   double *d_scr2 = nullptr; // NV*ncell scratch
   double *d_small = nullptr;  // NV*ncol + ncol scratch
   int *d_flag = nullptr;
+  int *d_work = nullptr;     // ncol: attempts of each column in the last adaptive launch
+  std::vector<int> h_work;   // host copy, used to order the work queues
   sedcol_params p{};
   bool have_params = false, have_geom = false, have_forcing = false;
   hipStream_t stream = nullptr;
@@ -275,6 +277,8 @@ int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
   HIPOK_C(hipMalloc(&h->d_diag, sizeof(sedcol_coldiag) * h->ncol));
   HIPOK_C(hipMalloc(&h->d_small, sizeof(double) * (NV + 1) * h->ncol));
   HIPOK_C(hipMalloc(&h->d_flag, sizeof(int) * 4));  // error flag + one work-queue counter per bin
+  HIPOK_C(hipMalloc(&h->d_work, sizeof(int) * h->ncol));
+  HIPOK_C(hipMemset(h->d_work, 0, sizeof(int) * h->ncol));
   HIPOK_C(hipMemset(h->d_conc, 0, NV * cb));
   HIPOK_C(hipMemset(h->d_cellacc, 0, 5 * cb));
   HIPOK_C(hipMemset(h->d_colacc, 0, sizeof(double) * NV * h->ncol));
@@ -292,7 +296,7 @@ int sedcol_destroy(sedcol_handle *h)
   if (h->stream) hipStreamSynchronize(h->stream);
   void *ptrs[] = {h->d_off, h->d_list[0], h->d_list[1], h->d_list[2], h->d_dz, h->d_dzc, h->d_phi, h->d_phiI,
                   h->d_bfac, h->d_biot, h->d_temp, h->d_cbw, h->d_fup, h->d_conc, h->d_cellacc, h->d_colacc,
-                  h->d_flux, h->d_diag, h->d_scr, h->d_scr2, h->d_small, h->d_flag};
+                  h->d_flux, h->d_diag, h->d_scr, h->d_scr2, h->d_small, h->d_flag, h->d_work};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (h->ev0) hipEventDestroy(h->ev0);
@@ -565,7 +569,7 @@ int sedcol_integrate(sedcol_handle *h, double t_start, double t_end)
   for (int b = 0; b < 3; ++b)
     if (!h->bins[b].empty()) {
       sc::launch_emp_adaptive(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, t_start, t_end, r0,
-                              r1, max_attempts, h->d_flag + 1 + b, h->d_flag);
+                              r1, max_attempts, h->d_flag + 1 + b, h->d_work, h->d_flag);
       ++launches;
     }
   HIPOK(hipGetLastError());
@@ -577,7 +581,22 @@ int sedcol_integrate(sedcol_handle *h, double t_start, double t_end)
   h->last_launches = launches;
   int flag = 0;
   if ((rc = read_flag(h, &flag))) return rc;
-  return flag_status(flag);
+  if ((rc = flag_status(flag))) return rc;
+  // Longest-first order for the next launch: the columns that took the most
+  // attempts in this one start first, so the launch does not end with a few
+  // long columns running on an otherwise idle device.  Results do not depend
+  // on the order (columns are independent).
+  h->h_work.resize(h->ncol);
+  HIPOK(hipMemcpyAsync(h->h_work.data(), h->d_work, sizeof(int) * h->ncol, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  for (int b = 0; b < 3; ++b) {
+    auto &v = h->bins[b];
+    if (v.size() < 2) continue;
+    std::stable_sort(v.begin(), v.end(), [&](int a, int c) { return h->h_work[a] > h->h_work[c]; });
+    HIPOK(hipMemcpyAsync(h->d_list[b], v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, h->stream));
+  }
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
 }
 
 int sedcol_step(sedcol_handle *h, double dt, int32_t method, double *conc_inout, double *flux_top_out)

@@ -332,7 +332,7 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(De
                                                                              sedcol_params p, double t_start,
                                                                              double t_end, int r0, int r1,
                                                                              long long max_attempts, int *next,
-                                                                             int *errflag)
+                                                                             int *work, int *errflag)
 {
   constexpr int G = BLOCK / LPC;
   __shared__ double sbc[G][2 * NV];
@@ -511,6 +511,7 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(De
         dg.zero_sink += (int64_t)zsum;
         dg.dt_next = dt;
         d.diag[c] = dg;
+        work[c] = attempts < 0x7fffffffll ? (int)attempts : 0x7fffffff;
       }
       have = false;
     }
@@ -770,14 +771,14 @@ static int emp_resident_blocks()
 }
 
 void launch_emp_adaptive(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p,
-                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *next,
+                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *next, int *work,
                          int *errflag)
 {
   SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, {
     const unsigned need = grid_for<LPC>(n).x;
     const unsigned cap = (unsigned)emp_resident_blocks<LPC, AR>();
     hipLaunchKernelGGL((k_emp_adaptive<LPC, AR>), dim3(need < cap ? need : cap), dim3(BLOCK), 0, st, d, list, n, p,
-                       t_start, t_end, r0, r1, max_attempts, next, errflag);
+                       t_start, t_end, r0, r1, max_attempts, next, work, errflag);
   }));
 }
 

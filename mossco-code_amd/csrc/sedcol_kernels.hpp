@@ -31,9 +31,10 @@ void launch_rhs(int lpc, hipStream_t st, const Dev &d, const int *list, int n, c
                 const double *conc, double *rhs, double *flux);
 void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *list, int n,
                   const sedcol_params &p, int rec, double dt, int nsteps, int *errflag);
-// next: a device int set to 0 before the launch (work-queue counter)
+// next: a device int set to 0 before the launch (work-queue counter);
+// work[c] receives the attempts column c took in this launch
 void launch_emp_adaptive(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p,
-                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *next,
+                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *next, int *work,
                          int *errflag);
 void launch_rhs_store(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                       double *r_out, double *s_out);

@@ -57,8 +57,9 @@ def build_fortran(lib: Path | None = None) -> Path | None:
     bdir.mkdir(exist_ok=True)
     _run([FC, "-O2", "-ffp-contract=off", "-c", FSRC / "sedcol_hip.F90", "-o", bdir / "sedcol_hip.o"], cwd=bdir)
     exe = FSRC / "fabmsed_hip"
+    rel = os.path.relpath(lib.parent, FSRC)
     _run([FC, "-O2", "-ffp-contract=off", "-I", bdir, FSRC / "fabmsed_hip.F90", bdir / "sedcol_hip.o",
-          "-o", exe, f"-L{lib.parent}", "-lsedcol", f"-Wl,-rpath,{lib.parent}"], cwd=bdir)
+          "-o", exe, f"-L{lib.parent}", "-lsedcol", f"-Wl,-rpath,$ORIGIN/{rel}"], cwd=bdir)
     return exe
 
 
