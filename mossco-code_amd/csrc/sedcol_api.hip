@@ -60,6 +60,10 @@ struct sedcol_handle {
   int *d_flag = nullptr;
   int *d_work = nullptr;     // ncol: attempts of each column in the last adaptive launch
   std::vector<int> h_work;   // host copy, used to order the work queues
+  Dev *d_devview = nullptr;  // device copy of dev(), read by the adaptive kernel
+  sedcol_params *d_params = nullptr;
+  Dev h_devview{};           // host staging of the copies (kept alive for async copies)
+  sedcol_params h_params{};
   sedcol_params p{};
   bool have_params = false, have_geom = false, have_forcing = false;
   hipStream_t stream = nullptr;
@@ -279,6 +283,8 @@ int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
   HIPOK_C(hipMalloc(&h->d_flag, sizeof(int) * 4));  // error flag + one work-queue counter per bin
   HIPOK_C(hipMalloc(&h->d_work, sizeof(int) * h->ncol));
   HIPOK_C(hipMemset(h->d_work, 0, sizeof(int) * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_devview, sizeof(Dev)));
+  HIPOK_C(hipMalloc(&h->d_params, sizeof(sedcol_params)));
   HIPOK_C(hipMemset(h->d_conc, 0, NV * cb));
   HIPOK_C(hipMemset(h->d_cellacc, 0, 5 * cb));
   HIPOK_C(hipMemset(h->d_colacc, 0, sizeof(double) * NV * h->ncol));
@@ -296,7 +302,8 @@ int sedcol_destroy(sedcol_handle *h)
   if (h->stream) hipStreamSynchronize(h->stream);
   void *ptrs[] = {h->d_off, h->d_list[0], h->d_list[1], h->d_list[2], h->d_dz, h->d_dzc, h->d_phi, h->d_phiI,
                   h->d_bfac, h->d_biot, h->d_temp, h->d_cbw, h->d_fup, h->d_conc, h->d_cellacc, h->d_colacc,
-                  h->d_flux, h->d_diag, h->d_scr, h->d_scr2, h->d_small, h->d_flag, h->d_work};
+                  h->d_flux, h->d_diag, h->d_scr, h->d_scr2, h->d_small, h->d_flag, h->d_work,
+                  h->d_devview, h->d_params};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (h->ev0) hipEventDestroy(h->ev0);
@@ -562,14 +569,18 @@ int sedcol_integrate(sedcol_handle *h, double t_start, double t_end)
     return fail(SEDCOL_EINVAL, "need finite t_start <= t_end");
   const int r0 = record_of(h, t_start), r1 = record_of(h, t_end);
   HIPOK(hipMemsetAsync(h->d_flag, 0, sizeof(int) * 4, h->stream));
-  const Dev d = h->dev();
+  h->h_devview = h->dev();
+  h->h_params = h->p;
+  HIPOK(hipMemcpyAsync(h->d_devview, &h->h_devview, sizeof(Dev), hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_params, &h->h_params, sizeof(sedcol_params), hipMemcpyHostToDevice, h->stream));
   int launches = 0;
   const long long max_attempts = 1ll << 40;
   HIPOK(hipEventRecord(h->ev0, h->stream));
   for (int b = 0; b < 3; ++b)
     if (!h->bins[b].empty()) {
-      sc::launch_emp_adaptive(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, t_start, t_end, r0,
-                              r1, max_attempts, h->d_flag + 1 + b, h->d_work, h->d_flag);
+      sc::launch_emp_adaptive(kLpc[b], h->stream, h->d_devview, h->d_list[b], (int)h->bins[b].size(), h->p,
+                              h->d_params, t_start, t_end, r0, r1, max_attempts, h->d_flag + 1 + b, h->d_work,
+                              h->d_flag);
       ++launches;
     }
   HIPOK(hipGetLastError());

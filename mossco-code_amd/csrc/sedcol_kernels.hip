@@ -328,12 +328,15 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 // with its own step size carried in diag.dt_next.  Each group exits when the
 // queue is empty, so every wave reaches the end.
 template <int LPC, int ARITH>
-__global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(Dev d, const int *list, int nlist,
-                                                                             sedcol_params p, double t_start,
-                                                                             double t_end, int r0, int r1,
-                                                                             long long max_attempts, int *next,
-                                                                             int *work, int *errflag)
+__global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
+    const Dev *dptr, const int *list, int nlist, const sedcol_params *__restrict__ pptr, double t_start,
+    double t_end, int r0, int r1, long long max_attempts, int *next, int *work, int *errflag)
 {
+  // Domain and parameters are read through pointers to device copies rather
+  // than passed by value: about 100 dwords of by-value arguments would be held
+  // in SGPRs for the whole kernel and spill.
+  const Dev &d = *dptr;
+  const sedcol_params &p = *pptr;
   constexpr int G = BLOCK / LPC;
   __shared__ double sbc[G][2 * NV];
   __shared__ double sF[G][2][NV];
@@ -770,15 +773,15 @@ static int emp_resident_blocks()
   return cached;
 }
 
-void launch_emp_adaptive(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p,
-                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *next, int *work,
-                         int *errflag)
+void launch_emp_adaptive(int lpc, hipStream_t st, const Dev *d_dev, const int *list, int n, const sedcol_params &p,
+                         const sedcol_params *d_params, double t_start, double t_end, int r0, int r1,
+                         long long max_attempts, int *next, int *work, int *errflag)
 {
   SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, {
     const unsigned need = grid_for<LPC>(n).x;
     const unsigned cap = (unsigned)emp_resident_blocks<LPC, AR>();
-    hipLaunchKernelGGL((k_emp_adaptive<LPC, AR>), dim3(need < cap ? need : cap), dim3(BLOCK), 0, st, d, list, n, p,
-                       t_start, t_end, r0, r1, max_attempts, next, work, errflag);
+    hipLaunchKernelGGL((k_emp_adaptive<LPC, AR>), dim3(need < cap ? need : cap), dim3(BLOCK), 0, st, d_dev, list, n,
+                       d_params, t_start, t_end, r0, r1, max_attempts, next, work, errflag);
   }));
 }
 

@@ -33,9 +33,11 @@ void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *
                   const sedcol_params &p, int rec, double dt, int nsteps, int *errflag);
 // next: a device int set to 0 before the launch (work-queue counter);
 // work[c] receives the attempts column c took in this launch
-void launch_emp_adaptive(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p,
-                         double t_start, double t_end, int r0, int r1, long long max_attempts, int *next, int *work,
-                         int *errflag);
+// d_dev / d_params: device copies of the domain view and the parameters
+// (p is the host copy, used to pick the kernel instance)
+void launch_emp_adaptive(int lpc, hipStream_t st, const Dev *d_dev, const int *list, int n, const sedcol_params &p,
+                         const sedcol_params *d_params, double t_start, double t_end, int r0, int r1,
+                         long long max_attempts, int *next, int *work, int *errflag);
 void launch_rhs_store(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                       double *r_out, double *s_out);
 void launch_ae_check(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double dt_red, double rcm,

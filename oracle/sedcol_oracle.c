@@ -396,10 +396,9 @@ static double col_psolve2(const double *c, long ks, const double *g, long gs,
   if (den) Sc = tree_prod(S, K);
   double pp = 0.0;
   double P[OC_KMAX], D[OC_KMAX];
-  /* Newton converges quadratically from below: once a correction is below
-   * 2^-26 p the next one would be at rounding level, so the iteration stops
-   * there (p below the root keeps every component positive, and the update
-   * c + dt p f conserves for any p). */
+  /* Newton converges quadratically from below; it stops once a correction
+   * is below 2^-20 p (the remaining error is then ~1e-12 p: p below the root
+   * keeps every component positive, and c + dt p f conserves for any p). */
   for (int it = 0; it < 60; ++it) {
     ++*newton;
     for (int k = 0; k < K; ++k) {
@@ -418,7 +417,7 @@ static double col_psolve2(const double *c, long ks, const double *g, long gs,
     double hp = Sc * D[0] - 1.0;
     double pn = pp - h / hp;
     if (!(pn > pp)) break;
-    int conv = (pn - pp) <= 0x1p-26 * pn;
+    int conv = (pn - pp) <= 0x1p-20 * pn;
     pp = pn;
     if (conv) break;
   }
