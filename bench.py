@@ -36,6 +36,13 @@ FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 DAY = 86400.0
 
+#: HBM bytes per k_emp_adaptive launch of the default workload (300 x 300,
+#: seed 0, fast form), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over
+#: three timed days: mean FETCH_SIZE 345 211 KiB x 2.00 + WRITE_SIZE
+#: 219 590 KiB x 1.00, the factors being what the counters read for a plain
+#: 256 MiB copy (FETCH_SIZE 131 084 KiB, WRITE_SIZE 262 144 KiB).
+DEFAULT_TRAFFIC_BYTES = (345211 * 2.0 + 219590 * 1.0) * 1024
+
 
 def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads):
     """Oracle (plain C, OpenMP over columns) on a sample of wet columns,
@@ -132,6 +139,8 @@ def main(argv=None):
     land_cells = np.repeat(dom.mask, K) != 0
     min_state = float(xs[:, ~land_cells].min())
     g.close()
+    default_workload = args.n == 300 and args.seed == 0 and args.arith == 1
+    traffic = DEFAULT_TRAFFIC_BYTES if default_workload else None
 
     out = {
         "metric": METRIC,
@@ -164,7 +173,9 @@ def main(argv=None):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)" if traffic else None,
+            "algorithmic_bytes_per_launch": flops.launch_bytes(K[wet], records=1),
             "algorithmic_flop_per_launch": fl / max(launches, 1),
             "avg_launch_ms": avg_launch_s * 1e3,
         },
