@@ -79,6 +79,103 @@ def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads):
     }
 
 
+def ragged_bench(args):
+    """Ragged batch: 1e5 columns, K in 10..60, depth 0.1-0.5 m, one forcing
+    record.  After an untimed warm-up the same span is integrated from the
+    same checkpoint by each schedule (one persistent launch per layer bin,
+    hipGraph-captured windows of --window attempts, one launch per attempt),
+    first with the default kernel shape, then with the autotuned one; the
+    final states must be bitwise equal across schedules (equal accuracy)."""
+    import hashlib
+
+    import torch
+
+    from sedcol import cases, capi, flops
+
+    torch.cuda.init()
+    t = time.perf_counter()
+    dom = cases.ragged_columns(ncol=args.ncol, seed=args.seed)
+    dom.params.arith = args.arith
+    gen_s = time.perf_counter() - t
+    K = dom.nlayers
+    span = args.span_hours * 3600.0
+    t_w = args.warmup * DAY
+    g = capi.SedimentColumns.from_domain(dom)
+    if t_w > 0:
+        g.integrate(0.0, t_w)
+    ck = g.save_checkpoint()
+    schedules = [
+        ("persistent", dict(window=0, graph_windows=0, check_every=1, concurrent_bins=0)),
+        ("persistent-concurrent-bins", dict(window=0, graph_windows=0, check_every=1, concurrent_bins=1)),
+        (f"graph-window-{args.window}", dict(window=args.window, graph_windows=args.graph_windows, check_every=1,
+                                              concurrent_bins=1)),
+        ("launch-per-attempt", dict(window=1, graph_windows=0, check_every=64, concurrent_bins=0)),
+    ]
+
+    def run_all(label):
+        rows = []
+        for name, sch in schedules:
+            g.load_checkpoint(ck)
+            g.set_schedule(**sch)
+            d0 = g.diagnostics()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g.integrate(t_w, t_w + span)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            ms, nl = g.last_timing()
+            d1 = g.diagnostics()
+            acc = d1["accepted"] - d0["accepted"]
+            att = acc + (d1["rejected"] - d0["rejected"])
+            newt = d1["newton"] - d0["newton"]
+            x = g.get_state()
+            rows.append({
+                "schedule": name, "shape": label, "wall_s": wall, "kernel_ms": ms, "launches": nl,
+                "column_steps_per_s": float(acc.sum()) / wall,
+                "tflops": flops.emp_flops(K, att, newt) / wall / 1e12,
+                "state_sha1": hashlib.sha1(x.tobytes()).hexdigest()[:16],
+            })
+            print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+        return rows, int(acc.sum())
+
+    rows, colsteps = run_all("default")
+    g.load_checkpoint(ck)
+    g.set_schedule(window=0)
+    t = time.perf_counter()
+    best, table = g.autotune(t_w, args.tune_span, repeats=2)
+    tune_s = time.perf_counter() - t
+    rows_t, _ = run_all("autotuned")
+    g.close()
+    allrows = rows + rows_t
+    same = len({r["state_sha1"] for r in allrows}) == 1
+    best_row = max(allrows, key=lambda r: r["column_steps_per_s"])
+    step_row = [r for r in rows_t if r["schedule"] == "launch-per-attempt"][0]
+    out = {
+        "metric": "FP64 sediment column-steps/s on 1 MI355X (ragged batch)",
+        "value": best_row["column_steps_per_s"],
+        "unit": "column-steps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"ragged {args.ncol} columns, K U{{10..60}}, depth U[0.1,0.5] m, adaptive EMP tol 1e-6, "
+                        f"dt in [1,3600] s; {args.warmup} warm-up day(s), then {args.span_hours} h timed",
+            "cells": int(dom.ncell), "seed": args.seed, "arith": "fast" if args.arith else "reference",
+        },
+        "best": {"schedule": best_row["schedule"], "shape": best_row["shape"]},
+        "speedup_vs_launch_per_attempt": best_row["column_steps_per_s"] / step_row["column_steps_per_s"],
+        "results_bitwise_equal": same,
+        "accepted_column_steps": colsteps,
+        "autotune": {"tuning": best, "seconds": tune_s, "span_s": args.tune_span,
+                     "ms_table_bin_lanes_workgroup": np.round(table, 3).tolist()},
+        "rows": allrows,
+        "input_generation_s": gen_s,
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if same else 1
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,12 +186,21 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--arith", type=int, default=1, help="0 reference operation order, 1 fast form")
+    ap.add_argument("--workload", choices=("shelf", "ragged"), default="shelf",
+                    help="shelf: the benchmark line; ragged: schedule/autotuner comparison on ragged columns")
+    ap.add_argument("--ncol", type=int, default=100000, help="ragged: columns")
+    ap.add_argument("--span-hours", type=float, default=24.0, help="ragged: timed span per schedule")
+    ap.add_argument("--window", type=int, default=100, help="ragged: attempts per column per window")
+    ap.add_argument("--graph-windows", type=int, default=8, help="ragged: windows per captured graph")
+    ap.add_argument("--tune-span", type=float, default=3600.0, help="ragged: autotuner trial span (s)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(columns are independent: see DESIGN.md for how the domain would shard).", file=sys.stderr)
         return 2
+    if args.workload == "ragged":
+        return ragged_bench(args)
 
     import torch
 

@@ -134,6 +134,32 @@ typedef struct {
   int32_t pad;
 } sedcol_coldiag;
 
+/* Schedule of adaptive integrations (sedcol_integrate).  No reference
+ * counterpart: the reference steps the whole domain with one dt per
+ * ode_solver call (solver_library.F90:80-189).  The default {0, 0, 1, 0}
+ * integrates each layer bin in one persistent launch.  Results are identical
+ * for every schedule. */
+typedef struct {
+  int32_t window;          /* attempts per column per launch; 0 = no limit */
+  int32_t graph_windows;   /* window > 0: windows captured in one hipGraph and
+                              replayed until every column is done; 0 = launch
+                              windows directly (one launch per step with
+                              window = 1) */
+  int32_t check_every;     /* direct windows launched between completion checks (>= 1) */
+  int32_t concurrent_bins; /* 1 = the layer bins run as concurrent branches */
+} sedcol_schedule;
+
+/* Kernel shape of the adaptive EMP kernel per layer bin (bins hold the wet
+ * columns with K <= 16, K <= 32 and K <= 64).  lanes_per_column: 16, 32 or
+ * 64 and at least the bin's largest K (one layer per lane); workgroup =
+ * lanes_per_column * columns_per_group threads, 64, 128 or 256.  Results are
+ * identical for every shape. */
+#define SEDCOL_NBIN 3
+typedef struct {
+  int32_t lanes_per_column[SEDCOL_NBIN];
+  int32_t columns_per_group[SEDCOL_NBIN];
+} sedcol_tuning;
+
 /* --- lifecycle ---------------------------------------------------------- */
 
 /* Replaces type_sed allocation in initialize (fabm_sediment_driver.F90:191-368)
@@ -209,8 +235,38 @@ int sedcol_run(sedcol_handle *h, double dt, int32_t nsteps, int32_t method,
 
 /* Adaptive EMP (method 6) on the device-resident state from t_start to
  * t_end under the forcing series (or the current forcing), each column with
- * its own step size; one persistent launch. */
+ * its own step size; persistent launches by the handle's schedule
+ * (sedcol_set_schedule). */
 int sedcol_integrate(sedcol_handle *h, double t_start, double t_end);
+
+/* Schedule and kernel shape of sedcol_integrate (no reference counterpart). */
+int sedcol_set_schedule(sedcol_handle *h, const sedcol_schedule *s);
+int sedcol_get_schedule(sedcol_handle *h, sedcol_schedule *s);
+int sedcol_set_tuning(sedcol_handle *h, const sedcol_tuning *t);
+int sedcol_get_tuning(sedcol_handle *h, sedcol_tuning *t);
+/* Autotuner (no reference counterpart): times sedcol_integrate over
+ * [t_start, t_start + t_span) for every supported shape of every bin, best
+ * of `repeats` runs, from the current state, which is restored after every
+ * trial and at the end.  The fastest shape per bin is installed and returned
+ * in *best; ms_table (optional, SEDCOL_NBIN x 3 x 3 doubles: bin, lanes
+ * 16/32/64, workgroup 64/128/256) receives every time, -1 where a shape does
+ * not apply. */
+int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t repeats,
+                    sedcol_tuning *best, double *ms_table);
+
+/* --- restart ------------------------------------------------------------ */
+
+/* Checkpoint of the device-resident integration state: state, budget
+ * accumulators, top fluxes and per-column diagnostics (including the step
+ * size carried into the next call), as one host byte blob of
+ * sedcol_checkpoint_size bytes.  Loading it into a handle of the same
+ * domain restores the state exactly, so an integration continued from it
+ * gives the same results as one never interrupted.  The counterpart of the
+ * restart read of fabm_sediment_component.F90:1328-1489, which restores the
+ * state only. */
+int sedcol_checkpoint_size(sedcol_handle *h, int64_t *bytes);
+int sedcol_save_checkpoint(sedcol_handle *h, void *buf, int64_t bytes);
+int sedcol_load_checkpoint(sedcol_handle *h, const void *buf, int64_t bytes);
 
 /* --- outputs ------------------------------------------------------------ */
 

@@ -57,8 +57,21 @@ struct sedcol_handle {
   double *d_scr = nullptr;  // NV*ncell scratch (rhs in/out, adaptive Euler rates)
   double *d_scr2 = nullptr; // NV*ncell scratch
   double *d_small = nullptr;  // NV*ncol + ncol scratch
-  int *d_flag = nullptr;
+  int *d_flag = nullptr;     // [0] error flags, [1..3] work-queue counters per bin, [4] windowed columns pending
+  int *h_flag = nullptr;     // pinned host copy of d_flag[0..4]
   int *d_work = nullptr;     // ncol: attempts of each column in the last adaptive launch
+  int *d_wrec = nullptr;     // ncol: adaptive position between windows (record, time left, attempts)
+  double *d_wrem = nullptr;
+  long long *d_watt = nullptr;
+  sc::EmpCtl *d_ctl = nullptr;
+  sc::EmpCtl h_ctl{};
+  sedcol_schedule sched{0, 0, 1, 0};
+  int lpc[3] = {16, 32, 64};  // lanes per column of each bin in the adaptive kernel
+  int blk[3] = {256, 256, 256};  // workgroup size of each bin in the adaptive kernel
+  hipStream_t aux[2] = {nullptr, nullptr};  // for concurrent bins
+  hipEvent_t evfork = nullptr, evjoin[2] = {nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   std::vector<int> h_work;   // host copy, used to order the work queues
   Dev *d_devview = nullptr;  // device copy of dev(), read by the adaptive kernel
   sedcol_params *d_params = nullptr;
@@ -96,6 +109,9 @@ struct sedcol_handle {
     d.colacc = d_colacc;
     d.flux = d_flux;
     d.diag = d_diag;
+    d.wrec = d_wrec;
+    d.wrem = d_wrem;
+    d.watt = d_watt;
     return d;
   }
 };
@@ -207,6 +223,149 @@ int record_of(const sedcol_handle *h, double t)
   return (int)q;
 }
 
+// smallest lanes per column that holds the largest column of bin b
+int bin_min_lpc(const sedcol_handle *h, int b)
+{
+  int kmax = 1;
+  for (int c : h->bins[b]) kmax = std::max(kmax, h->off[c + 1] - h->off[c]);
+  return kmax <= 16 ? 16 : kmax <= 32 ? 32 : 64;
+}
+
+int check_tuning(const sedcol_handle *h, const sedcol_tuning *t)
+{
+  for (int b = 0; b < SEDCOL_NBIN; ++b) {
+    const int l = t->lanes_per_column[b], g = t->columns_per_group[b];
+    if (l != 16 && l != 32 && l != 64) return fail(SEDCOL_EINVAL, "lanes_per_column must be 16, 32 or 64");
+    if (l < bin_min_lpc(h, b)) return fail(SEDCOL_EINVAL, "lanes_per_column below the bin's largest column");
+    const int w = l * g;
+    if (g < 1 || (w != 64 && w != 128 && w != 256))
+      return fail(SEDCOL_EINVAL, "lanes_per_column * columns_per_group must be 64, 128 or 256");
+  }
+  return SEDCOL_OK;
+}
+
+void drop_graph(sedcol_handle *h)
+{
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->graph_exec = nullptr;
+  h->graph = nullptr;
+}
+
+// One adaptive launch per non-empty bin (only bin `only` when only >= 0):
+// in order on the handle's stream, or as concurrent branches forked from it.
+// Work-queue counters and the pending count must be zero beforehand.
+int launch_bins(sedcol_handle *h, int only, int *launches)
+{
+  int list[3], nb = 0;
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty() && (only < 0 || only == b)) list[nb++] = b;
+  const bool fork = h->sched.concurrent_bins && nb > 1;
+  if (fork) HIPOK(hipEventRecord(h->evfork, h->stream));
+  for (int i = 0; i < nb; ++i) {
+    const int b = list[i];
+    hipStream_t st = h->stream;
+    if (fork && i > 0) {
+      st = h->aux[i - 1];
+      HIPOK(hipStreamWaitEvent(st, h->evfork, 0));
+    }
+    if (!sc::launch_emp_adaptive(h->lpc[b], h->blk[b], st, h->d_devview, h->d_list[b], (int)h->bins[b].size(), h->p,
+                                 h->d_params, h->d_ctl, h->d_flag + 1 + b, h->d_flag + 4, h->d_work, h->d_flag))
+      return fail(SEDCOL_EINVAL, "unsupported lanes per column / workgroup size");
+    ++*launches;
+    if (fork && i > 0) {
+      HIPOK(hipEventRecord(h->evjoin[i - 1], st));
+      HIPOK(hipStreamWaitEvent(h->stream, h->evjoin[i - 1], 0));
+    }
+  }
+  HIPOK(hipGetLastError());
+  return SEDCOL_OK;
+}
+
+// pending count and error flags after the work queued so far
+int read_progress(sedcol_handle *h, int *flag, int *pending)
+{
+  HIPOK(hipMemcpyAsync(h->h_flag, h->d_flag, sizeof(int) * 5, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  *flag = h->h_flag[0];
+  *pending = h->h_flag[4];
+  return SEDCOL_OK;
+}
+
+// graph of sched.graph_windows windows: (reset counters, one launch per bin)
+int capture_graph(sedcol_handle *h)
+{
+  drop_graph(h);
+  // device queries of the launcher (resident blocks) outside the capture
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty() && sc::emp_resident_blocks(h->lpc[b], h->blk[b], h->p) <= 0)
+      return fail(SEDCOL_EINVAL, "unsupported lanes per column / workgroup size");
+  int launches = 0, rc = SEDCOL_OK;
+  HIPOK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  for (int w = 0; w < h->sched.graph_windows && rc == SEDCOL_OK; ++w) {
+    sc::launch_fill_int(h->stream, h->d_flag + 1, 4, 0);
+    rc = launch_bins(h, -1, &launches);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(SEDCOL_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  h->graph = g;
+  HIPOK(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+  return SEDCOL_OK;
+}
+
+// Adaptive EMP over [t_start, t_end) for every bin (or bin `only`), by the
+// handle's schedule.  Sets *launches to the adaptive kernel launches made.
+int run_adaptive(sedcol_handle *h, double t_start, double t_end, int only, int *launches)
+{
+  const int r0 = record_of(h, t_start), r1 = record_of(h, t_end);
+  const sedcol_schedule &sc_ = h->sched;
+  const bool windowed = sc_.window > 0;
+  HIPOK(hipMemsetAsync(h->d_flag, 0, sizeof(int) * 8, h->stream));
+  h->h_devview = h->dev();
+  h->h_params = h->p;
+  h->h_ctl.t_start = t_start;
+  h->h_ctl.t_end = t_end;
+  h->h_ctl.max_attempts = 1ll << 40;
+  h->h_ctl.r0 = r0;
+  h->h_ctl.r1 = r1;
+  h->h_ctl.window = windowed ? sc_.window : 0;
+  h->h_ctl.resume = windowed ? 1 : 0;
+  HIPOK(hipMemcpyAsync(h->d_devview, &h->h_devview, sizeof(Dev), hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_params, &h->h_params, sizeof(sedcol_params), hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_ctl, &h->h_ctl, sizeof(sc::EmpCtl), hipMemcpyHostToDevice, h->stream));
+  *launches = 0;
+  if (!windowed) return launch_bins(h, only, launches);
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty() && (only < 0 || only == b))
+      sc::launch_emp_begin(h->stream, h->d_list[b], (int)h->bins[b].size(), r0, h->d_wrec, h->d_wrem, h->d_watt);
+  HIPOK(hipGetLastError());
+  int rc, flag = 0, pending = 0;
+  const bool graph = sc_.graph_windows > 0 && only < 0;
+  if (graph && !h->graph_exec && (rc = capture_graph(h))) return rc;
+  int nb = 0;
+  for (int b = 0; b < 3; ++b) nb += !h->bins[b].empty();
+  for (;;) {
+    if (graph) {
+      HIPOK(hipGraphLaunch(h->graph_exec, h->stream));
+      *launches += sc_.graph_windows * nb;
+    } else {
+      for (int w = 0; w < sc_.check_every; ++w) {
+        sc::launch_fill_int(h->stream, h->d_flag + 1, 4, 0);
+        if ((rc = launch_bins(h, only, launches))) return rc;
+      }
+    }
+    if ((rc = read_progress(h, &flag, &pending))) return rc;
+    if (pending == 0 || flag) break;
+  }
+  return SEDCOL_OK;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -280,8 +439,19 @@ int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
   HIPOK_C(hipMalloc(&h->d_flux, sizeof(double) * NV * h->ncol));
   HIPOK_C(hipMalloc(&h->d_diag, sizeof(sedcol_coldiag) * h->ncol));
   HIPOK_C(hipMalloc(&h->d_small, sizeof(double) * (NV + 1) * h->ncol));
-  HIPOK_C(hipMalloc(&h->d_flag, sizeof(int) * 4));  // error flag + one work-queue counter per bin
+  HIPOK_C(hipMalloc(&h->d_flag, sizeof(int) * 8));  // error flag, one work-queue counter per bin, pending
+  HIPOK_C(hipMemset(h->d_flag, 0, sizeof(int) * 8));
+  HIPOK_C(hipHostMalloc((void **)&h->h_flag, sizeof(int) * 8, hipHostMallocDefault));
   HIPOK_C(hipMalloc(&h->d_work, sizeof(int) * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_wrec, sizeof(int) * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_wrem, sizeof(double) * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_watt, sizeof(long long) * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_ctl, sizeof(sc::EmpCtl)));
+  for (int a = 0; a < 2; ++a) {
+    HIPOK_C(hipStreamCreateWithFlags(&h->aux[a], hipStreamNonBlocking));
+    HIPOK_C(hipEventCreateWithFlags(&h->evjoin[a], hipEventDisableTiming));
+  }
+  HIPOK_C(hipEventCreateWithFlags(&h->evfork, hipEventDisableTiming));
   HIPOK_C(hipMemset(h->d_work, 0, sizeof(int) * h->ncol));
   HIPOK_C(hipMalloc(&h->d_devview, sizeof(Dev)));
   HIPOK_C(hipMalloc(&h->d_params, sizeof(sedcol_params)));
@@ -303,9 +473,17 @@ int sedcol_destroy(sedcol_handle *h)
   void *ptrs[] = {h->d_off, h->d_list[0], h->d_list[1], h->d_list[2], h->d_dz, h->d_dzc, h->d_phi, h->d_phiI,
                   h->d_bfac, h->d_biot, h->d_temp, h->d_cbw, h->d_fup, h->d_conc, h->d_cellacc, h->d_colacc,
                   h->d_flux, h->d_diag, h->d_scr, h->d_scr2, h->d_small, h->d_flag, h->d_work,
-                  h->d_devview, h->d_params};
+                  h->d_devview, h->d_params, h->d_wrec, h->d_wrem, h->d_watt, h->d_ctl};
   for (void *q : ptrs)
     if (q) hipFree(q);
+  if (h->h_flag) hipHostFree(h->h_flag);
+  if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  for (int a = 0; a < 2; ++a) {
+    if (h->aux[a]) hipStreamDestroy(h->aux[a]);
+    if (h->evjoin[a]) hipEventDestroy(h->evjoin[a]);
+  }
+  if (h->evfork) hipEventDestroy(h->evfork);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -330,6 +508,7 @@ int sedcol_set_params(sedcol_handle *h, const sedcol_params *p)
   if (p->reserved != 0) return fail(SEDCOL_EINVAL, "reserved must be 0");
   h->p = *p;
   h->have_params = true;
+  drop_graph(h);  // the kernel instance depends on the arithmetic form
   return SEDCOL_OK;
 }
 
@@ -567,23 +746,9 @@ int sedcol_integrate(sedcol_handle *h, double t_start, double t_end)
   if (rc) return rc;
   if (!std::isfinite(t_start) || !std::isfinite(t_end) || !(t_end >= t_start))
     return fail(SEDCOL_EINVAL, "need finite t_start <= t_end");
-  const int r0 = record_of(h, t_start), r1 = record_of(h, t_end);
-  HIPOK(hipMemsetAsync(h->d_flag, 0, sizeof(int) * 4, h->stream));
-  h->h_devview = h->dev();
-  h->h_params = h->p;
-  HIPOK(hipMemcpyAsync(h->d_devview, &h->h_devview, sizeof(Dev), hipMemcpyHostToDevice, h->stream));
-  HIPOK(hipMemcpyAsync(h->d_params, &h->h_params, sizeof(sedcol_params), hipMemcpyHostToDevice, h->stream));
   int launches = 0;
-  const long long max_attempts = 1ll << 40;
   HIPOK(hipEventRecord(h->ev0, h->stream));
-  for (int b = 0; b < 3; ++b)
-    if (!h->bins[b].empty()) {
-      sc::launch_emp_adaptive(kLpc[b], h->stream, h->d_devview, h->d_list[b], (int)h->bins[b].size(), h->p,
-                              h->d_params, t_start, t_end, r0, r1, max_attempts, h->d_flag + 1 + b, h->d_work,
-                              h->d_flag);
-      ++launches;
-    }
-  HIPOK(hipGetLastError());
+  if ((rc = run_adaptive(h, t_start, t_end, -1, &launches))) return rc;
   HIPOK(hipEventRecord(h->ev1, h->stream));
   HIPOK(hipEventSynchronize(h->ev1));
   float ms = 0.f;
@@ -610,6 +775,146 @@ int sedcol_integrate(sedcol_handle *h, double t_start, double t_end)
   return SEDCOL_OK;
 }
 
+int sedcol_set_schedule(sedcol_handle *h, const sedcol_schedule *s)
+{
+  if (!h || !s) return fail(SEDCOL_EINVAL, "null argument");
+  if (s->window < 0) return fail(SEDCOL_EINVAL, "window must be >= 0");
+  if (s->graph_windows < 0 || s->graph_windows > 4096) return fail(SEDCOL_EINVAL, "graph_windows outside 0..4096");
+  if (s->check_every < 1) return fail(SEDCOL_EINVAL, "check_every must be >= 1");
+  if (s->concurrent_bins != 0 && s->concurrent_bins != 1) return fail(SEDCOL_EINVAL, "concurrent_bins must be 0 or 1");
+  h->sched = *s;
+  drop_graph(h);
+  return SEDCOL_OK;
+}
+
+int sedcol_get_schedule(sedcol_handle *h, sedcol_schedule *s)
+{
+  if (!h || !s) return fail(SEDCOL_EINVAL, "null argument");
+  *s = h->sched;
+  return SEDCOL_OK;
+}
+
+int sedcol_set_tuning(sedcol_handle *h, const sedcol_tuning *t)
+{
+  if (!h || !t) return fail(SEDCOL_EINVAL, "null argument");
+  int rc = check_tuning(h, t);
+  if (rc) return rc;
+  for (int b = 0; b < SEDCOL_NBIN; ++b) {
+    h->lpc[b] = t->lanes_per_column[b];
+    h->blk[b] = t->lanes_per_column[b] * t->columns_per_group[b];
+  }
+  drop_graph(h);
+  return SEDCOL_OK;
+}
+
+int sedcol_get_tuning(sedcol_handle *h, sedcol_tuning *t)
+{
+  if (!h || !t) return fail(SEDCOL_EINVAL, "null argument");
+  for (int b = 0; b < SEDCOL_NBIN; ++b) {
+    t->lanes_per_column[b] = h->lpc[b];
+    t->columns_per_group[b] = h->blk[b] / h->lpc[b];
+  }
+  return SEDCOL_OK;
+}
+
+int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t repeats, sedcol_tuning *best,
+                    double *ms_table)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!best) return fail(SEDCOL_EINVAL, "null argument");
+  if (!std::isfinite(t_start) || !(t_span > 0.0) || !std::isfinite(t_span))
+    return fail(SEDCOL_EINVAL, "need finite t_start and t_span > 0");
+  if (repeats < 1 || repeats > 100) return fail(SEDCOL_EINVAL, "repeats outside 1..100");
+  static const int kL[3] = {16, 32, 64}, kW[3] = {64, 128, 256};
+  if (ms_table)
+    for (int i = 0; i < SEDCOL_NBIN * 9; ++i) ms_table[i] = -1.0;
+  // snapshot of everything an integration changes; restored before every
+  // trial and at the end, so tuning leaves the handle's state as it was
+  const size_t nc = (size_t)h->ncol, ncell = (size_t)h->ncell;
+  struct Snap { void *dst; const void *src; size_t bytes; };
+  std::vector<Snap> snaps = {
+      {nullptr, h->d_conc, sizeof(double) * NV * ncell}, {nullptr, h->d_cellacc, sizeof(double) * 5 * ncell},
+      {nullptr, h->d_colacc, sizeof(double) * NV * nc},  {nullptr, h->d_flux, sizeof(double) * NV * nc},
+      {nullptr, h->d_diag, sizeof(sedcol_coldiag) * nc}, {nullptr, h->d_work, sizeof(int) * nc}};
+  auto release = [&]() {
+    for (auto &q : snaps)
+      if (q.dst) hipFree(q.dst);
+  };
+  for (auto &q : snaps) {
+    if (hipMalloc(&q.dst, q.bytes) != hipSuccess) {
+      release();
+      return fail(SEDCOL_EHIP, "sedcol_autotune: out of device memory for the state snapshot");
+    }
+    if (hipMemcpyAsync(q.dst, q.src, q.bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) {
+      release();
+      return fail(SEDCOL_EHIP, "sedcol_autotune: snapshot copy failed");
+    }
+  }
+  auto restore = [&]() -> int {
+    for (auto &q : snaps) HIPOK(hipMemcpyAsync(const_cast<void *>(q.src), q.dst, q.bytes, hipMemcpyDeviceToDevice,
+                                               h->stream));
+    return SEDCOL_OK;
+  };
+  const sedcol_schedule saved_sched = h->sched;
+  int saved_lpc[3], saved_blk[3];
+  std::memcpy(saved_lpc, h->lpc, sizeof saved_lpc);
+  std::memcpy(saved_blk, h->blk, sizeof saved_blk);
+  h->sched = sedcol_schedule{0, 0, 1, 0};  // one launch per trial
+  int best_l[3], best_w[3];
+  std::memcpy(best_l, h->lpc, sizeof best_l);
+  std::memcpy(best_w, h->blk, sizeof best_w);
+  for (int b = 0; b < 3 && rc == SEDCOL_OK; ++b) {
+    if (h->bins[b].empty()) continue;
+    const int lmin = bin_min_lpc(h, b);
+    double best_ms = INFINITY;
+    for (int il = 0; il < 3 && rc == SEDCOL_OK; ++il) {
+      if (kL[il] < lmin) continue;
+      for (int iw = 0; iw < 3 && rc == SEDCOL_OK; ++iw) {
+        if (kW[iw] < kL[il]) continue;
+        h->lpc[b] = kL[il];
+        h->blk[b] = kW[iw];
+        double tmin = INFINITY;
+        for (int r = 0; r < repeats && rc == SEDCOL_OK; ++r) {
+          int launches = 0, flag = 0;
+          if ((rc = restore())) break;
+          if (hipEventRecord(h->ev0, h->stream) != hipSuccess) { rc = fail(SEDCOL_EHIP, "event record"); break; }
+          if ((rc = run_adaptive(h, t_start, t_start + t_span, b, &launches))) break;
+          if (hipEventRecord(h->ev1, h->stream) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess) {
+            rc = fail(SEDCOL_EHIP, "event synchronize");
+            break;
+          }
+          float ms = 0.f;
+          hipEventElapsedTime(&ms, h->ev0, h->ev1);
+          if ((rc = read_flag(h, &flag))) break;
+          if ((rc = flag_status(flag))) break;
+          tmin = std::min(tmin, (double)ms);
+        }
+        if (rc) break;
+        if (ms_table) ms_table[b * 9 + il * 3 + iw] = tmin;
+        if (tmin < best_ms) {
+          best_ms = tmin;
+          best_l[b] = kL[il];
+          best_w[b] = kW[iw];
+        }
+      }
+    }
+  }
+  int rc2 = restore();
+  if (rc2 == SEDCOL_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc2 = fail(SEDCOL_EHIP, "synchronize");
+  release();
+  h->sched = saved_sched;
+  if (rc || rc2) {
+    std::memcpy(h->lpc, saved_lpc, sizeof saved_lpc);
+    std::memcpy(h->blk, saved_blk, sizeof saved_blk);
+    return rc ? rc : rc2;
+  }
+  std::memcpy(h->lpc, best_l, sizeof best_l);
+  std::memcpy(h->blk, best_w, sizeof best_w);
+  drop_graph(h);
+  return sedcol_get_tuning(h, best);
+}
+
 int sedcol_step(sedcol_handle *h, double dt, int32_t method, double *conc_inout, double *flux_top_out)
 {
   int rc = check_handle(h, true);
@@ -627,6 +932,77 @@ int sedcol_step(sedcol_handle *h, double dt, int32_t method, double *conc_inout,
   if (rc) return rc;
   if (rc2) return rc2;
   if (flux_top_out) return sedcol_get_fluxes(h, flux_top_out);
+  return SEDCOL_OK;
+}
+
+namespace {
+struct CkHeader {
+  char magic[8];
+  int64_t ncol, ncell, nvar, version;
+};
+constexpr char kCkMagic[8] = {'S', 'E', 'D', 'C', 'O', 'L', 'C', 'K'};
+
+// device arrays of a checkpoint, in blob order
+std::vector<std::pair<void *, size_t>> ck_parts(sedcol_handle *h)
+{
+  const size_t nc = (size_t)h->ncol, ncell = (size_t)h->ncell;
+  return {{h->d_conc, sizeof(double) * NV * ncell},  {h->d_cellacc, sizeof(double) * 5 * ncell},
+          {h->d_colacc, sizeof(double) * NV * nc},   {h->d_flux, sizeof(double) * NV * nc},
+          {h->d_diag, sizeof(sedcol_coldiag) * nc}};
+}
+
+int64_t ck_bytes(sedcol_handle *h)
+{
+  int64_t n = sizeof(CkHeader);
+  for (auto &q : ck_parts(h)) n += (int64_t)q.second;
+  return n;
+}
+}  // namespace
+
+int sedcol_checkpoint_size(sedcol_handle *h, int64_t *bytes)
+{
+  if (!h || !bytes) return fail(SEDCOL_EINVAL, "null argument");
+  *bytes = ck_bytes(h);
+  return SEDCOL_OK;
+}
+
+int sedcol_save_checkpoint(sedcol_handle *h, void *buf, int64_t bytes)
+{
+  if (!h || !buf) return fail(SEDCOL_EINVAL, "null argument");
+  if (bytes < ck_bytes(h)) return fail(SEDCOL_EINVAL, "checkpoint buffer too small (sedcol_checkpoint_size)");
+  CkHeader hd;
+  std::memcpy(hd.magic, kCkMagic, 8);
+  hd.ncol = h->ncol;
+  hd.ncell = h->ncell;
+  hd.nvar = NV;
+  hd.version = 1;
+  std::memcpy(buf, &hd, sizeof hd);
+  char *p = static_cast<char *>(buf) + sizeof hd;
+  for (auto &q : ck_parts(h)) {
+    HIPOK(hipMemcpyAsync(p, q.first, q.second, hipMemcpyDeviceToHost, h->stream));
+    p += q.second;
+  }
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int sedcol_load_checkpoint(sedcol_handle *h, const void *buf, int64_t bytes)
+{
+  if (!h || !buf) return fail(SEDCOL_EINVAL, "null argument");
+  if (bytes < (int64_t)sizeof(CkHeader)) return fail(SEDCOL_EINVAL, "checkpoint too short");
+  CkHeader hd;
+  std::memcpy(&hd, buf, sizeof hd);
+  if (std::memcmp(hd.magic, kCkMagic, 8) != 0 || hd.version != 1)
+    return fail(SEDCOL_EINVAL, "not a sedcol checkpoint");
+  if (hd.ncol != h->ncol || hd.ncell != h->ncell || hd.nvar != NV)
+    return fail(SEDCOL_EINVAL, "checkpoint of a different domain");
+  if (bytes < ck_bytes(h)) return fail(SEDCOL_EINVAL, "checkpoint too short");
+  const char *p = static_cast<const char *>(buf) + sizeof hd;
+  for (auto &q : ck_parts(h)) {
+    HIPOK(hipMemcpyAsync(q.first, p, q.second, hipMemcpyHostToDevice, h->stream));
+    p += q.second;
+  }
+  HIPOK(hipStreamSynchronize(h->stream));
   return SEDCOL_OK;
 }
 

@@ -327,17 +327,27 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 // [t0 + r dt_rec, t0 + (r+1) dt_rec), the first and last extend outward),
 // with its own step size carried in diag.dt_next.  Each group exits when the
 // queue is empty, so every wave reaches the end.
-template <int LPC, int ARITH>
-__global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
-    const Dev *dptr, const int *list, int nlist, const sedcol_params *__restrict__ pptr, double t_start,
-    double t_end, int r0, int r1, long long max_attempts, int *next, int *work, int *errflag)
+//
+// Windows: with ctl.window > 0 a column stops after that many attempts in
+// one launch and saves its position (record, time left in the record,
+// attempts so far) in d.wrec / d.wrem / d.watt; *pending counts the columns
+// so stopped.  With ctl.resume = 1 every column continues from its saved
+// position (k_emp_begin sets the start position) and columns whose record
+// is past r1 are skipped.  A column's operations are the same however its
+// attempts are split over launches, so windowed and single-launch runs give
+// identical results.
+template <int LPC, int ARITH, int BLK>
+__global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
+    const Dev *dptr, const int *list, int nlist, const sedcol_params *__restrict__ pptr,
+    const EmpCtl *__restrict__ cptr, int *next, int *pending, int *work, int *errflag)
 {
-  // Domain and parameters are read through pointers to device copies rather
-  // than passed by value: about 100 dwords of by-value arguments would be held
-  // in SGPRs for the whole kernel and spill.
+  // Domain, parameters and span are read through pointers to device copies
+  // rather than passed by value: about 100 dwords of by-value arguments would
+  // be held in SGPRs for the whole kernel and spill.
   const Dev &d = *dptr;
   const sedcol_params &p = *pptr;
-  constexpr int G = BLOCK / LPC;
+  const EmpCtl &ctl = *cptr;
+  constexpr int G = BLK / LPC;
   __shared__ double sbc[G][2 * NV];
   __shared__ double sF[G][2][NV];
   __shared__ int sslot[G];
@@ -345,6 +355,7 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
   const bool own = l < NV;  // lane l accumulates the top flux of variable l
   double(*F)[NV] = sF[g];
   const double *bc = sbc[g];
+  const int r1 = ctl.r1;
 
   // state of the group's current column
   int c = 0, r = 0;
@@ -356,6 +367,7 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
   sedcol_coldiag dg;
   int64_t zs = 0;
   long long attempts = 0;
+  int wn = 0;
   double dt = 0.0, rem = 0.0;
   bool have = false, act = false;
 
@@ -367,6 +379,16 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
       __builtin_amdgcn_wave_barrier();
       if (s >= nlist) break;
       c = list[s];
+      if (ctl.resume) {
+        r = d.wrec[c];
+        if (r > r1) continue;  // finished in an earlier window
+        rem = d.wrem[c];
+        attempts = d.watt[c];
+      } else {
+        r = ctl.r0 - 1;
+        rem = 0.0;
+        attempts = 0;
+      }
       setup_lane(d, c, l, L, Gm, cell);
       act = L.active;
 #pragma unroll
@@ -379,10 +401,13 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
       dt = dg.dt_next;
       if (!(dt > 0.0)) dt = p.emp_dt_max;
       zs = 0;
-      attempts = 0;
-      r = r0 - 1;
-      rem = 0.0;
+      wn = 0;
       have = true;
+      if (rem > 0.0) {
+        // resumed inside record r
+        load_forcing<ARITH>(d, p, c, r, Gm, L, sbc[g]);
+        __builtin_amdgcn_wave_barrier();
+      }
     }
     bool finish = false;
     if (!(rem > 0.0)) {
@@ -393,23 +418,27 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
       } else {
         const double lo = (r == 0) ? -INFINITY : d.t0 + (double)r * d.dt_rec;
         const double hi = (r == d.nrec - 1) ? INFINITY : d.t0 + (double)(r + 1) * d.dt_rec;
-        const double a = t_start > lo ? t_start : lo;
-        const double b = t_end < hi ? t_end : hi;
+        const double a = ctl.t_start > lo ? ctl.t_start : lo;
+        const double b = ctl.t_end < hi ? ctl.t_end : hi;
         rem = b - a;
         load_forcing<ARITH>(d, p, c, r, Gm, L, sbc[g]);
         __builtin_amdgcn_wave_barrier();
         continue;
       }
     }
+    bool pause = false;
     if (!finish) {
-      if (attempts >= max_attempts) {
+      if (attempts >= ctl.max_attempts) {
         dg.status = 2;
         if (l == 0) atomicOr(errflag, 2);
         finish = true;
+      } else if (ctl.window > 0 && wn >= ctl.window) {
+        pause = true;
       }
     }
-    if (!finish) {
+    if (!finish && !pause) {
       ++attempts;
+      ++wn;
       const bool clipped = !(dt < rem);
       const double h = clipped ? rem : dt;
       double f0[NV], c1[NV];
@@ -497,7 +526,7 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
         if (dt > p.emp_dt_max) dt = p.emp_dt_max;
       }
     }
-    if (finish) {
+    if (finish || pause) {
 #pragma unroll
       for (int n = 0; n < NV; ++n)
         if (act) d.conc[n * d.ncell + cell] = x[n];
@@ -515,6 +544,11 @@ __global__ __launch_bounds__(BLOCK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
         dg.dt_next = dt;
         d.diag[c] = dg;
         work[c] = attempts < 0x7fffffffll ? (int)attempts : 0x7fffffff;
+        // finished (or stopped by an error): record past r1
+        d.wrec[c] = pause ? r : r1 + 1;
+        d.wrem[c] = rem;
+        d.watt[c] = attempts;
+        if (pause) atomicAdd(pending, 1);
       }
       have = false;
     }
@@ -758,31 +792,88 @@ void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *
 }
 
 // blocks of a work-queue kernel that are resident at once on this device
-template <int LPC, int AR>
-static int emp_resident_blocks()
+template <int LPC, int AR, int BLK>
+static int emp_resident()
 {
   static int cached = 0;
   if (cached == 0) {
     int dev = 0, per_cu = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive<LPC, AR>, BLOCK, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive<LPC, AR, BLK>, BLK, 0) != hipSuccess)
       per_cu = 1;
     cached = (per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount;
   }
   return cached;
 }
 
-void launch_emp_adaptive(int lpc, hipStream_t st, const Dev *d_dev, const int *list, int n, const sedcol_params &p,
-                         const sedcol_params *d_params, double t_start, double t_end, int r0, int r1,
-                         long long max_attempts, int *next, int *work, int *errflag)
+#define SC_DISPATCH_BLK(blk, ...)                                \
+  switch (blk) {                                                 \
+    case 64: { constexpr int BLK = 64; __VA_ARGS__; } break;     \
+    case 128: { constexpr int BLK = 128; __VA_ARGS__; } break;   \
+    case 256: { constexpr int BLK = 256; __VA_ARGS__; } break;   \
+    default: return false;                                       \
+  }
+
+static bool emp_shape_ok(int lpc, int blk)
 {
-  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, {
-    const unsigned need = grid_for<LPC>(n).x;
-    const unsigned cap = (unsigned)emp_resident_blocks<LPC, AR>();
-    hipLaunchKernelGGL((k_emp_adaptive<LPC, AR>), dim3(need < cap ? need : cap), dim3(BLOCK), 0, st, d_dev, list, n,
-                       d_params, t_start, t_end, r0, r1, max_attempts, next, work, errflag);
-  }));
+  return (lpc == 16 || lpc == 32 || lpc == 64) && (blk == 64 || blk == 128 || blk == 256);
+}
+
+int emp_resident_blocks(int lpc, int blk, const sedcol_params &p)
+{
+  if (!emp_shape_ok(lpc, blk)) return 0;
+  int out = 0;
+  auto body = [&]() -> bool {
+    SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, out = emp_resident<LPC, AR, BLK>())));
+    return true;
+  };
+  body();
+  return out;
+}
+
+bool launch_emp_adaptive(int lpc, int blk, hipStream_t st, const Dev *d_dev, const int *list, int n,
+                         const sedcol_params &p, const sedcol_params *d_params, const EmpCtl *d_ctl, int *next,
+                         int *pending, int *work, int *errflag)
+{
+  if (!emp_shape_ok(lpc, blk)) return false;
+  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, {
+    const int G = BLK / LPC;
+    const unsigned need = (unsigned)((n + G - 1) / G);
+    const unsigned cap = (unsigned)emp_resident<LPC, AR, BLK>();
+    hipLaunchKernelGGL((k_emp_adaptive<LPC, AR, BLK>), dim3(need < cap ? need : cap), dim3(BLK), 0, st, d_dev,
+                       list, n, d_params, d_ctl, next, pending, work, errflag);
+  })));
+  return true;
+}
+
+__global__ void k_emp_begin(const int *list, int n, int r0, int *wrec, double *wrem, long long *watt)
+{
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = list[i];
+    wrec[c] = r0 - 1;
+    wrem[c] = 0.0;
+    watt[c] = 0;
+  }
+}
+
+__global__ void k_fill_int(int *p, int n, int v)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_emp_begin(hipStream_t st, const int *list, int n, int r0, int *wrec, double *wrem, long long *watt)
+{
+  if (n <= 0) return;
+  const int blocks = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
+  hipLaunchKernelGGL(k_emp_begin, dim3(blocks), dim3(256), 0, st, list, n, r0, wrec, wrem, watt);
+}
+
+void launch_fill_int(hipStream_t st, int *p, int n, int v)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
 }
 
 void launch_rhs_store(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,

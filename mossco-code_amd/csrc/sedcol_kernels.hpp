@@ -25,19 +25,42 @@ struct Dev {
   double *colacc;   // NV*ncol: time-integrated top flux
   double *flux;     // NV*ncol: top flux of the last right-hand side
   sedcol_coldiag *diag;
+  // adaptive integration position of each column between windows
+  int *wrec;        // ncol: current forcing record (past the last one: done)
+  double *wrem;     // ncol: time left in that record
+  long long *watt;  // ncol: attempts since the start of the span
+};
+
+// Span of one adaptive integration, read by the kernel through a pointer
+struct EmpCtl {
+  double t_start, t_end;
+  long long max_attempts;
+  int r0, r1;   // first and last forcing record touched by the span
+  int window;   // attempts per column per launch; <= 0: no limit
+  int resume;   // 1: continue every column from d.wrec / d.wrem / d.watt
 };
 
 void launch_rhs(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                 const double *conc, double *rhs, double *flux);
 void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *list, int n,
                   const sedcol_params &p, int rec, double dt, int nsteps, int *errflag);
-// next: a device int set to 0 before the launch (work-queue counter);
-// work[c] receives the attempts column c took in this launch
-// d_dev / d_params: device copies of the domain view and the parameters
-// (p is the host copy, used to pick the kernel instance)
-void launch_emp_adaptive(int lpc, hipStream_t st, const Dev *d_dev, const int *list, int n, const sedcol_params &p,
-                         const sedcol_params *d_params, double t_start, double t_end, int r0, int r1,
-                         long long max_attempts, int *next, int *work, int *errflag);
+// Adaptive EMP over the span in *d_ctl for the n columns of list, lpc lanes
+// per column and blk threads per workgroup.  next: a device int set to 0
+// before the launch (work-queue counter); pending: incremented once per
+// column stopped by the window limit; work[c] receives the attempts column c
+// has taken since the start of the span.  d_dev / d_params: device copies of
+// the domain view and the parameters (p is the host copy, used to pick the
+// kernel instance).  Returns false for an unsupported (lpc, blk).
+bool launch_emp_adaptive(int lpc, int blk, hipStream_t st, const Dev *d_dev, const int *list, int n,
+                         const sedcol_params &p, const sedcol_params *d_params, const EmpCtl *d_ctl, int *next,
+                         int *pending, int *work, int *errflag);
+// blocks of that kernel resident at once on the current device
+int emp_resident_blocks(int lpc, int blk, const sedcol_params &p);
+// start position of every listed column: record r0 - 1, nothing left in it,
+// no attempts
+void launch_emp_begin(hipStream_t st, const int *list, int n, int r0, int *wrec, double *wrem, long long *watt);
+// v into p[0..n-1] (a kernel, so that it can be captured in a graph)
+void launch_fill_int(hipStream_t st, int *p, int n, int v);
 void launch_rhs_store(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                       double *r_out, double *s_out);
 void launch_ae_check(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double dt_red, double rcm,

@@ -37,7 +37,10 @@ EXPORTS = (
     "sedcol_step", "sedcol_run", "sedcol_integrate", "sedcol_get_fluxes", "sedcol_get_budget",
     "sedcol_reset_budget", "sedcol_get_inventory", "sedcol_get_diagnostics", "sedcol_reset_diagnostics",
     "sedcol_residual", "sedcol_get_last_min_dt", "sedcol_synchronize", "sedcol_last_timing",
+    "sedcol_set_schedule", "sedcol_get_schedule", "sedcol_set_tuning", "sedcol_get_tuning", "sedcol_autotune",
+    "sedcol_checkpoint_size", "sedcol_save_checkpoint", "sedcol_load_checkpoint",
 )
+NBIN = 3
 
 
 class SedcolError(RuntimeError):
@@ -61,6 +64,15 @@ class ColDiag(ctypes.Structure):
                 ("zero_sink", ctypes.c_int64), ("newton", ctypes.c_int64), ("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double),
                 ("dt_minacc", ctypes.c_double), ("p_min", ctypes.c_double), ("status", ctypes.c_int32),
                 ("pad", ctypes.c_int32)]
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("window", ctypes.c_int32), ("graph_windows", ctypes.c_int32), ("check_every", ctypes.c_int32),
+                ("concurrent_bins", ctypes.c_int32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("lanes_per_column", ctypes.c_int32 * 3), ("columns_per_group", ctypes.c_int32 * 3)]
 
 
 DIAG_DTYPE = np.dtype([("accepted", np.int64), ("rejected", np.int64), ("forced", np.int64),
@@ -109,6 +121,14 @@ def lib() -> ctypes.CDLL:
             "sedcol_get_last_min_dt": (ctypes.c_int, [vp, pd]),
             "sedcol_synchronize": (ctypes.c_int, [vp]),
             "sedcol_last_timing": (ctypes.c_int, [vp, pd, ctypes.POINTER(ctypes.c_int32)]),
+            "sedcol_set_schedule": (ctypes.c_int, [vp, ctypes.POINTER(Schedule)]),
+            "sedcol_get_schedule": (ctypes.c_int, [vp, ctypes.POINTER(Schedule)]),
+            "sedcol_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+            "sedcol_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+            "sedcol_autotune": (ctypes.c_int, [vp, dbl, dbl, i32, ctypes.POINTER(Tuning), pd]),
+            "sedcol_checkpoint_size": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
+            "sedcol_save_checkpoint": (ctypes.c_int, [vp, vp, ctypes.c_int64]),
+            "sedcol_load_checkpoint": (ctypes.c_int, [vp, vp, ctypes.c_int64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -231,6 +251,49 @@ class SedimentColumns:
 
     def integrate(self, t_start, t_end):
         check(lib().sedcol_integrate(self._h, float(t_start), float(t_end)))
+
+    # -- schedule and kernel shape -------------------------------------
+    def set_schedule(self, window=0, graph_windows=0, check_every=1, concurrent_bins=0):
+        s = Schedule(int(window), int(graph_windows), int(check_every), int(concurrent_bins))
+        check(lib().sedcol_set_schedule(self._h, ctypes.byref(s)))
+
+    def get_schedule(self) -> dict:
+        s = Schedule()
+        check(lib().sedcol_get_schedule(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in Schedule._fields_}
+
+    def set_tuning(self, lanes_per_column, columns_per_group):
+        t = Tuning((ctypes.c_int32 * 3)(*lanes_per_column), (ctypes.c_int32 * 3)(*columns_per_group))
+        check(lib().sedcol_set_tuning(self._h, ctypes.byref(t)))
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        check(lib().sedcol_get_tuning(self._h, ctypes.byref(t)))
+        return {"lanes_per_column": list(t.lanes_per_column), "columns_per_group": list(t.columns_per_group)}
+
+    def autotune(self, t_start, t_span, repeats=2):
+        """Time every kernel shape per layer bin, install the fastest.
+
+        Returns (tuning dict, ms table of shape (3 bins, 3 lanes 16/32/64,
+        3 workgroups 64/128/256), -1 where a shape does not apply)."""
+        t = Tuning()
+        table = np.empty((NBIN, 3, 3))
+        check(lib().sedcol_autotune(self._h, float(t_start), float(t_span), int(repeats), ctypes.byref(t),
+                                    _pd(table)))
+        return ({"lanes_per_column": list(t.lanes_per_column), "columns_per_group": list(t.columns_per_group)},
+                table)
+
+    # -- restart -------------------------------------------------------
+    def save_checkpoint(self) -> bytes:
+        n = ctypes.c_int64()
+        check(lib().sedcol_checkpoint_size(self._h, ctypes.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        check(lib().sedcol_save_checkpoint(self._h, ctypes.c_void_p(buf.ctypes.data), n.value))
+        return buf.tobytes()
+
+    def load_checkpoint(self, blob: bytes):
+        buf = np.frombuffer(blob, dtype=np.uint8)
+        check(lib().sedcol_load_checkpoint(self._h, ctypes.c_void_p(buf.ctypes.data), buf.size))
 
     # -- outputs -------------------------------------------------------
     def fluxes(self):
