@@ -254,6 +254,26 @@ int sedcol_get_tuning(sedcol_handle *h, sedcol_tuning *t);
 int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t repeats,
                     sedcol_tuning *best, double *ms_table);
 
+/* Per-column spin-up to steady state, the replacement of the single-column
+ * presimulation plus broadcast of fabm_sediment_component.F90:557-632 (the
+ * reference integrates one column for presimulation_years with a fixed dt of
+ * 3600 s and copies it to every wet column).  Every wet column is integrated
+ * by adaptive EMP from its resident state under the current forcing, in
+ * chunks of `chunk` seconds from t_start, until its steady-state residual
+ * (sedcol_residual with cfloor) is below tol or t_max seconds have passed.
+ * A column is retired from the integration as soon as it is converged.
+ * flux_ref (nvar x ncol, optional): with it, a column also needs its top
+ * fluxes F within flux_rtol * max(|flux_ref|, flux_floor[c]) of flux_ref
+ * for every variable to count as converged.
+ * t_conv (ncol, optional): time after t_start at which the column converged,
+ * -1 if it did not (and for land columns).  resid (ncol, optional): final
+ * residual.  *n_unconverged (optional): wet columns still above tol.
+ * No reference counterpart for the per-column form. */
+int sedcol_spinup(sedcol_handle *h, double t_start, double chunk, double t_max, double tol,
+                  const double *cfloor, const double *flux_ref, double flux_rtol,
+                  const double *flux_floor, double *t_conv, double *resid,
+                  int32_t *n_unconverged);
+
 /* --- restart ------------------------------------------------------------ */
 
 /* Checkpoint of the device-resident integration state: state, budget

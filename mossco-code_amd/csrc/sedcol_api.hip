@@ -365,7 +365,69 @@ int run_adaptive(sedcol_handle *h, double t_start, double t_end, int only, int *
   return SEDCOL_OK;
 }
 
+// NV*ncell scratch arrays, allocated on first use
+int ensure_scratch(sedcol_handle *h)
+{
+  if (!h->d_scr) HIPOK(hipMalloc(&h->d_scr, sizeof(double) * NV * (size_t)h->ncell));
+  if (!h->d_scr2) HIPOK(hipMalloc(&h->d_scr2, sizeof(double) * NV * (size_t)h->ncell));
+  return SEDCOL_OK;
+}
 
+// upload the current bins as the work lists (after a change of the active set)
+int upload_bins(sedcol_handle *h)
+{
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty())
+      HIPOK(hipMemcpyAsync(h->d_list[b], h->bins[b].data(), sizeof(int) * h->bins[b].size(),
+                           hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  drop_graph(h);  // the graph holds the list lengths
+  return SEDCOL_OK;
+}
+struct CkHeader {
+  char magic[8];
+  int64_t ncol, ncell, nvar, version;
+};
+constexpr char kCkMagic[8] = {'S', 'E', 'D', 'C', 'O', 'L', 'C', 'K'};
+
+// device arrays of a checkpoint, in blob order
+std::vector<std::pair<void *, size_t>> ck_parts(sedcol_handle *h)
+{
+  const size_t nc = (size_t)h->ncol, ncell = (size_t)h->ncell;
+  return {{h->d_conc, sizeof(double) * NV * ncell},  {h->d_cellacc, sizeof(double) * 5 * ncell},
+          {h->d_colacc, sizeof(double) * NV * nc},   {h->d_flux, sizeof(double) * NV * nc},
+          {h->d_diag, sizeof(sedcol_coldiag) * nc}};
+}
+
+int64_t ck_bytes(sedcol_handle *h)
+{
+  int64_t n = sizeof(CkHeader);
+  for (auto &q : ck_parts(h)) n += (int64_t)q.second;
+  return n;
+}
+// steady-state residual of the listed columns (and their top fluxes, NV x
+// ncol, when flux is not null) into host arrays
+int residual_flux(sedcol_handle *h, const double *cfloor, double *resid, double *flux)
+{
+  int rc;
+  if ((flux || h->ncol < NV) && (rc = ensure_scratch(h))) return rc;
+  double *d_cf = h->d_small + (size_t)NV * h->ncol;  // NV slots needed
+  double *d_res = h->d_small;
+  if (h->ncol < NV) d_cf = h->d_scr2;
+  double *d_fl = flux ? h->d_scr : nullptr;  // NV*ncell >= NV*ncol
+  HIPOK(hipMemcpyAsync(d_cf, cfloor, sizeof(double) * NV, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemsetAsync(d_res, 0, sizeof(double) * h->ncol, h->stream));
+  if (d_fl) HIPOK(hipMemsetAsync(d_fl, 0, sizeof(double) * NV * h->ncol, h->stream));
+  const Dev d = h->dev();
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty())
+      sc::launch_residual(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, 0, d_cf, d_res, d_fl);
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(resid, d_res, sizeof(double) * h->ncol, hipMemcpyDeviceToHost, h->stream));
+  if (d_fl) HIPOK(hipMemcpyAsync(flux, d_fl, sizeof(double) * NV * h->ncol, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -603,12 +665,6 @@ int sedcol_get_state(sedcol_handle *h, double *conc)
   return SEDCOL_OK;
 }
 
-static int ensure_scratch(sedcol_handle *h)
-{
-  if (!h->d_scr) HIPOK(hipMalloc(&h->d_scr, sizeof(double) * NV * (size_t)h->ncell));
-  if (!h->d_scr2) HIPOK(hipMalloc(&h->d_scr2, sizeof(double) * NV * (size_t)h->ncell));
-  return SEDCOL_OK;
-}
 
 int sedcol_rhs(sedcol_handle *h, const double *conc, double *rhs, double *flux_top_out)
 {
@@ -935,29 +991,74 @@ int sedcol_step(sedcol_handle *h, double dt, int32_t method, double *conc_inout,
   return SEDCOL_OK;
 }
 
-namespace {
-struct CkHeader {
-  char magic[8];
-  int64_t ncol, ncell, nvar, version;
-};
-constexpr char kCkMagic[8] = {'S', 'E', 'D', 'C', 'O', 'L', 'C', 'K'};
-
-// device arrays of a checkpoint, in blob order
-std::vector<std::pair<void *, size_t>> ck_parts(sedcol_handle *h)
+int sedcol_spinup(sedcol_handle *h, double t_start, double chunk, double t_max, double tol, const double *cfloor,
+                  const double *flux_ref, double flux_rtol, const double *flux_floor, double *t_conv, double *resid,
+                  int32_t *n_unconverged)
 {
-  const size_t nc = (size_t)h->ncol, ncell = (size_t)h->ncell;
-  return {{h->d_conc, sizeof(double) * NV * ncell},  {h->d_cellacc, sizeof(double) * 5 * ncell},
-          {h->d_colacc, sizeof(double) * NV * nc},   {h->d_flux, sizeof(double) * NV * nc},
-          {h->d_diag, sizeof(sedcol_coldiag) * nc}};
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!cfloor) return fail(SEDCOL_EINVAL, "null cfloor");
+  if (!std::isfinite(t_start) || !(chunk > 0.0) || !std::isfinite(chunk) || !(t_max >= 0.0) ||
+      !std::isfinite(t_max) || !(tol > 0.0))
+    return fail(SEDCOL_EINVAL, "need finite t_start, chunk > 0, t_max >= 0, tol > 0");
+  if (flux_ref && (!flux_floor || !(flux_rtol >= 0.0)))
+    return fail(SEDCOL_EINVAL, "a flux criterion needs flux_floor and flux_rtol >= 0");
+  const int nc = h->ncol;
+  std::vector<double> tc(nc, -1.0), res(nc, 0.0);
+  std::vector<double> fl(flux_ref ? (size_t)NV * nc : 0);
+  auto converged = [&](int c) {
+    if (!(res[c] < tol)) return false;
+    if (!flux_ref) return true;
+    for (int n = 0; n < NV; ++n) {
+      const double fr = flux_ref[(size_t)n * nc + c];
+      const double sc = std::max(std::fabs(fr), flux_floor[c]);
+      if (!(std::fabs(fl[(size_t)n * nc + c] - fr) <= flux_rtol * sc)) return false;
+    }
+    return true;
+  };
+  std::vector<int> full[3];
+  for (int b = 0; b < 3; ++b) full[b] = h->bins[b];
+  std::vector<char> active(nc, 0);
+  for (int b = 0; b < 3; ++b)
+    for (int c : full[b]) active[c] = 1;
+  double t = t_start;
+  const double t_end = t_start + t_max;
+  for (;;) {
+    if ((rc = residual_flux(h, cfloor, res.data(), flux_ref ? fl.data() : nullptr))) break;
+    int left = 0;
+    for (int b = 0; b < 3; ++b) {
+      std::vector<int> keep;
+      keep.reserve(h->bins[b].size());
+      for (int c : h->bins[b]) {
+        if (converged(c)) {
+          tc[c] = t - t_start;
+          active[c] = 0;
+        } else {
+          keep.push_back(c);
+        }
+      }
+      left += (int)keep.size();
+      h->bins[b].swap(keep);
+    }
+    if (left == 0 || !(t < t_end)) break;
+    if ((rc = upload_bins(h))) break;
+    const double t1 = std::min(t + chunk, t_end);
+    if ((rc = sedcol_integrate(h, t, t1))) break;
+    t = t1;
+  }
+  // every wet column back in the work lists, final residual of all
+  for (int b = 0; b < 3; ++b) h->bins[b] = full[b];
+  int rc2 = upload_bins(h);
+  if (!rc && !rc2) rc2 = residual_flux(h, cfloor, res.data(), nullptr);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  int32_t un = 0;
+  for (int c = 0; c < nc; ++c) un += active[c] ? 1 : 0;
+  if (t_conv) std::memcpy(t_conv, tc.data(), sizeof(double) * nc);
+  if (resid) std::memcpy(resid, res.data(), sizeof(double) * nc);
+  if (n_unconverged) *n_unconverged = un;
+  return SEDCOL_OK;
 }
-
-int64_t ck_bytes(sedcol_handle *h)
-{
-  int64_t n = sizeof(CkHeader);
-  for (auto &q : ck_parts(h)) n += (int64_t)q.second;
-  return n;
-}
-}  // namespace
 
 int sedcol_checkpoint_size(sedcol_handle *h, int64_t *bytes)
 {
@@ -1075,22 +1176,7 @@ int sedcol_residual(sedcol_handle *h, const double *cfloor, double *resid)
   int rc = check_handle(h, true);
   if (rc) return rc;
   if (!cfloor || !resid) return fail(SEDCOL_EINVAL, "null argument");
-  double *d_cf = h->d_small + (size_t)NV * h->ncol;  // ncol >= 1 slots; NV needed
-  double *d_res = h->d_small;
-  if (h->ncol < NV) {
-    if ((rc = ensure_scratch(h))) return rc;
-    d_cf = h->d_scr2;
-  }
-  HIPOK(hipMemcpyAsync(d_cf, cfloor, sizeof(double) * NV, hipMemcpyHostToDevice, h->stream));
-  HIPOK(hipMemsetAsync(d_res, 0, sizeof(double) * h->ncol, h->stream));
-  const Dev d = h->dev();
-  for (int b = 0; b < 3; ++b)
-    if (!h->bins[b].empty())
-      sc::launch_residual(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, 0, d_cf, d_res);
-  HIPOK(hipGetLastError());
-  HIPOK(hipMemcpyAsync(resid, d_res, sizeof(double) * h->ncol, hipMemcpyDeviceToHost, h->stream));
-  HIPOK(hipStreamSynchronize(h->stream));
-  return SEDCOL_OK;
+  return residual_flux(h, cfloor, resid, nullptr);
 }
 
 int sedcol_get_last_min_dt(sedcol_handle *h, double *last_min_dt)

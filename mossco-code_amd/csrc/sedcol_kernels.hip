@@ -714,7 +714,7 @@ __global__ __launch_bounds__(BLOCK) void k_inventory(Dev d, const int *list, int
 
 template <int LPC, int ARITH>
 __global__ __launch_bounds__(BLOCK) void k_residual(Dev d, const int *list, int nlist, sedcol_params p, int rec,
-                                                    const double *cfloor, double *resid)
+                                                    const double *cfloor, double *resid, double *flux)
 {
   constexpr int G = BLOCK / LPC;
   __shared__ double sbc[G][2 * NV];
@@ -740,6 +740,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual(Dev d, const int *list, int 
     const double den = ax > cfloor[n] ? ax : cfloor[n];
     const double e = fabs(r) / den;
     if (L.active && (e > m || e != e)) m = e;
+    if (l == 0 && flux) flux[(long long)n * d.ncol + c] = F;
   });
   m = lg_max<LPC>(m);
   if (l == 0) resid[c] = m;
@@ -916,10 +917,10 @@ void launch_inventory(int lpc, hipStream_t st, const Dev &d, const int *list, in
 }
 
 void launch_residual(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
-                     const double *cfloor, double *resid)
+                     const double *cfloor, double *resid, double *flux)
 {
   SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_residual<LPC, AR>), grid_for<LPC>(n), dim3(BLOCK),
-                                                               0, st, d, list, n, p, rec, cfloor, resid)));
+                                                               0, st, d, list, n, p, rec, cfloor, resid, flux)));
 }
 
 }  // namespace sc

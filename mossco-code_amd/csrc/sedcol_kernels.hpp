@@ -71,6 +71,6 @@ void launch_clamp(int lpc, hipStream_t st, const Dev &d, const int *list, int n,
 void launch_budget(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double *budget);
 void launch_inventory(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double *inv);
 void launch_residual(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
-                     const double *cfloor, double *resid);
+                     const double *cfloor, double *resid, double *flux = nullptr);
 
 }  // namespace sc

@@ -47,6 +47,15 @@ def build_library(out: Path | None = None, extra=()) -> Path:
     return out
 
 
+def build_surrogate(out: Path | None = None, extra=()) -> Path:
+    """libsedsurr.so: the spin-up surrogate's training and inference kernels."""
+    out = Path(out) if out else PKG / "libsedsurr.so"
+    tmp = out.with_suffix(".so.tmp")
+    _run([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-I{CSRC}", CSRC / "sedsurr.hip", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_fortran(lib: Path | None = None) -> Path | None:
     """ISO_C_BINDING module + the fabmsed-style driver program."""
     if not shutil.which(FC) and not Path(FC).exists():
@@ -65,6 +74,7 @@ def build_fortran(lib: Path | None = None) -> Path | None:
 
 def main(argv=None):
     build_library()
+    build_surrogate()
     if (FSRC / "sedcol_hip.F90").exists():
         build_fortran()
 

@@ -38,7 +38,7 @@ EXPORTS = (
     "sedcol_reset_budget", "sedcol_get_inventory", "sedcol_get_diagnostics", "sedcol_reset_diagnostics",
     "sedcol_residual", "sedcol_get_last_min_dt", "sedcol_synchronize", "sedcol_last_timing",
     "sedcol_set_schedule", "sedcol_get_schedule", "sedcol_set_tuning", "sedcol_get_tuning", "sedcol_autotune",
-    "sedcol_checkpoint_size", "sedcol_save_checkpoint", "sedcol_load_checkpoint",
+    "sedcol_checkpoint_size", "sedcol_save_checkpoint", "sedcol_load_checkpoint", "sedcol_spinup",
 )
 NBIN = 3
 
@@ -126,6 +126,8 @@ def lib() -> ctypes.CDLL:
             "sedcol_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
             "sedcol_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
             "sedcol_autotune": (ctypes.c_int, [vp, dbl, dbl, i32, ctypes.POINTER(Tuning), pd]),
+            "sedcol_spinup": (ctypes.c_int, [vp, dbl, dbl, dbl, dbl, pd, pd, dbl, pd, pd, pd,
+                                             ctypes.POINTER(ctypes.c_int32)]),
             "sedcol_checkpoint_size": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
             "sedcol_save_checkpoint": (ctypes.c_int, [vp, vp, ctypes.c_int64]),
             "sedcol_load_checkpoint": (ctypes.c_int, [vp, vp, ctypes.c_int64]),
@@ -282,6 +284,25 @@ class SedimentColumns:
                                     _pd(table)))
         return ({"lanes_per_column": list(t.lanes_per_column), "columns_per_group": list(t.columns_per_group)},
                 table)
+
+    def spinup(self, t_start, chunk, t_max, tol, cfloor, flux_ref=None, flux_rtol=0.01, flux_floor=None):
+        """Per-column adaptive-EMP spin-up; returns (t_conv, resid,
+        n_unconverged): time after t_start at which each column converged
+        (residual below tol and, with flux_ref (NV, ncol), top fluxes within
+        flux_rtol * max(|flux_ref|, flux_floor)), -1 if never / land; final
+        residual per column."""
+        cf = _c(cfloor)
+        tc = np.empty(self.ncol)
+        rs = np.empty(self.ncol)
+        nu = ctypes.c_int32()
+        fr = ff = None
+        if flux_ref is not None:
+            fr = _c(flux_ref).reshape(NVAR, self.ncol)
+            ff = _c(np.broadcast_to(0.0 if flux_floor is None else flux_floor, (self.ncol,)))
+        check(lib().sedcol_spinup(self._h, float(t_start), float(chunk), float(t_max), float(tol), _pd(cf),
+                                  _pd(fr) if fr is not None else None, float(flux_rtol),
+                                  _pd(ff) if ff is not None else None, _pd(tc), _pd(rs), ctypes.byref(nu)))
+        return tc, rs, nu.value
 
     # -- restart -------------------------------------------------------
     def save_checkpoint(self) -> bytes:

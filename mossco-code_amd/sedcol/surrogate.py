@@ -1,0 +1,413 @@
+"""Spin-up surrogate: binding of libsedsurr.so, training driver, warm start.
+
+The network maps the 12 column inputs of sedcol.spinup (INPUTS) to the
+log10 steady-state profiles on 30 layers (330 values); it is trained on
+steady states from cold spin-ups (sedcol.spinup.generate) by the library's
+own BF16 MFMA kernels, and its prediction, interpolated to each column's
+layers, is the warm start that sedcol_spinup continues from.  Columns whose
+warm start does not reach the steady-state criterion within the warm budget
+fall back to a cold start.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_NAME = "libsedsurr.so"
+SHAPE = (12, 512, 3, 330)  # n_in, n_hidden, hidden layers, n_out
+
+#: every symbol include/sedsurr.h declares
+EXPORTS = ("sedsurr_create", "sedsurr_destroy", "sedsurr_last_error", "sedsurr_num_params", "sedsurr_set_params",
+           "sedsurr_get_params", "sedsurr_set_data", "sedsurr_train", "sedsurr_loss_grad", "sedsurr_predict",
+           "sedsurr_last_timing")
+
+
+class Shape(ctypes.Structure):
+    _fields_ = [("n_in", ctypes.c_int32), ("n_hidden", ctypes.c_int32), ("n_layers", ctypes.c_int32),
+                ("n_out", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Adam(ctypes.Structure):
+    _fields_ = [("lr", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("eps", ctypes.c_float)]
+
+
+class SurrogateError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sedsurr status {code}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def lib_path() -> Path:
+    return Path(os.environ.get("SEDSURR_LIB", HERE / LIB_NAME))
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not path.exists():
+            raise OSError(f"{path} not found: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(str(path))
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        pf = ctypes.POINTER(ctypes.c_float)
+        sig = {
+            "sedsurr_create": (ctypes.c_int, [ctypes.POINTER(Shape), ctypes.POINTER(vp)]),
+            "sedsurr_destroy": (ctypes.c_int, [vp]),
+            "sedsurr_last_error": (ctypes.c_char_p, []),
+            "sedsurr_num_params": (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
+            "sedsurr_set_params": (ctypes.c_int, [vp, pf]),
+            "sedsurr_get_params": (ctypes.c_int, [vp, pf]),
+            "sedsurr_set_data": (ctypes.c_int, [vp, pf, pf, i64]),
+            "sedsurr_train": (ctypes.c_int, [vp, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(Adam), pf]),
+            "sedsurr_loss_grad": (ctypes.c_int, [vp, pf, pf, i32, pf, pf]),
+            "sedsurr_predict": (ctypes.c_int, [vp, pf, i64, pf]),
+            "sedsurr_last_timing": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(code):
+    if code != 0:
+        msg = lib().sedsurr_last_error()
+        raise SurrogateError(code, msg.decode() if msg else "")
+
+
+def _pf(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Network:
+    """A device-resident MLP (libsedsurr.so handle)."""
+
+    def __init__(self, shape=SHAPE, max_batch=4096):
+        self.shape = tuple(int(v) for v in shape)
+        s = Shape(*self.shape, int(max_batch), 0)
+        h = ctypes.c_void_p()
+        check(lib().sedsurr_create(ctypes.byref(s), ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_int64()
+        check(lib().sedsurr_num_params(self._h, ctypes.byref(n)))
+        self.num_params = n.value
+        self.max_batch = int(max_batch)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sedsurr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.num_params
+        check(lib().sedsurr_set_params(self._h, _pf(p)))
+
+    def get_params(self):
+        p = np.empty(self.num_params, dtype=np.float32)
+        check(lib().sedsurr_get_params(self._h, _pf(p)))
+        return p
+
+    def set_data(self, X, Y):
+        X, Y = _f32(X), _f32(Y)
+        self._n = X.shape[0]
+        check(lib().sedsurr_set_data(self._h, _pf(X), _pf(Y), X.shape[0]))
+
+    def train(self, rows, batch, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        steps = rows.size // batch
+        loss = np.empty(steps, dtype=np.float32)
+        a = Adam(lr, beta1, beta2, eps)
+        check(lib().sedsurr_train(self._h, steps, int(batch), rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                  ctypes.byref(a), _pf(loss)))
+        return loss
+
+    def loss_grad(self, X, Y):
+        X, Y = _f32(X), _f32(Y)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.empty(self.num_params, dtype=np.float32)
+        check(lib().sedsurr_loss_grad(self._h, _pf(X), _pf(Y), X.shape[0], _pf(loss), _pf(g)))
+        return float(loss[0]), g
+
+    def predict(self, X):
+        X = _f32(X)
+        Y = np.empty((X.shape[0], self.shape[3]), dtype=np.float32)
+        check(lib().sedsurr_predict(self._h, _pf(X), X.shape[0], _pf(Y)))
+        return Y
+
+    def last_timing(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int32()
+        check(lib().sedsurr_last_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+def init_params(rng: np.random.Generator, shape=SHAPE) -> np.ndarray:
+    """Glorot-uniform weights, zero biases, in the flat layout of
+    sedsurr_get_params."""
+    n_in, n_hidden, n_layers, n_out = shape
+    dims = [n_in] + [n_hidden] * n_layers + [n_out]
+    parts = []
+    for fi, fo in zip(dims[:-1], dims[1:]):
+        a = np.sqrt(6.0 / (fi + fo))
+        parts.append(rng.uniform(-a, a, size=fi * fo))
+        parts.append(np.zeros(fo))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def flops_per_sample(shape=SHAPE) -> float:
+    """Multiply-adds x 2 of one forward pass."""
+    n_in, n_hidden, n_layers, n_out = shape
+    dims = [n_in] + [n_hidden] * n_layers + [n_out]
+    return 2.0 * sum(fi * fo for fi, fo in zip(dims[:-1], dims[1:]))
+
+
+@dataclass
+class Model:
+    """Trained parameters and the normalisation of inputs and targets."""
+    params: np.ndarray
+    x_mean: np.ndarray
+    x_std: np.ndarray
+    y_mean: np.ndarray
+    y_std: np.ndarray
+    shape: tuple = SHAPE
+
+    def save(self, path):
+        np.savez(path, params=self.params, x_mean=self.x_mean, x_std=self.x_std, y_mean=self.y_mean,
+                 y_std=self.y_std, shape=np.asarray(self.shape))
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path)  # plain arrays, no pickles
+        return cls(z["params"], z["x_mean"], z["x_std"], z["y_mean"], z["y_std"], tuple(int(v) for v in z["shape"]))
+
+
+def fit(X, Y, *, epochs=400, batch=4096, lr=1e-3, lr_final=1e-5, val_fraction=0.1, seed=0, log=None,
+        net: Network | None = None):
+    """Train on (X (n x 12), Y (n x 330)) with Adam on shuffled minibatches,
+    learning rate decayed geometrically from lr to lr_final over the epochs.
+    Returns (Model, history dict)."""
+    import time
+
+    rng = np.random.Generator(np.random.Philox(seed))
+    X = np.asarray(X, dtype=np.float64)
+    Y = np.asarray(Y, dtype=np.float64)
+    n = X.shape[0]
+    perm = rng.permutation(n)
+    nval = int(round(val_fraction * n))
+    iv, it = perm[:nval], perm[nval:]
+    x_mean, x_std = X[it].mean(0), X[it].std(0) + 1e-12
+    y_mean, y_std = Y[it].mean(0), Y[it].std(0) + 1e-6
+    Xn = (X - x_mean) / x_std
+    Yn = (Y - y_mean) / y_std
+    own = net is None
+    net = net or Network(SHAPE, max_batch=batch)
+    try:
+        net.set_params(init_params(rng, net.shape))
+        net.set_data(Xn[it], Yn[it])
+        ntr = it.size
+        b = min(batch, ntr - ntr % 64 if ntr >= 64 else ntr)
+        steps_per_epoch = max(1, ntr // b)
+        hist = {"epoch": [], "train_loss": [], "val_loss": [], "lr": []}
+        train_ms = 0.0
+        launches = 0
+        t0 = time.perf_counter()
+        for ep in range(epochs):
+            cur = lr * (lr_final / lr) ** (ep / max(1, epochs - 1))
+            rows = np.concatenate([rng.permutation(ntr)[: steps_per_epoch * b]])
+            loss = net.train(rows, b, lr=cur)
+            ms, nl = net.last_timing()
+            train_ms += ms
+            launches += nl
+            if ep % 10 == 0 or ep == epochs - 1:
+                vl = float(np.mean((net.predict(Xn[iv]) - Yn[iv]) ** 2)) if nval else float("nan")
+                hist["epoch"].append(ep)
+                hist["train_loss"].append(float(loss.mean()))
+                hist["val_loss"].append(vl)
+                hist["lr"].append(cur)
+                if log:
+                    log(f"epoch {ep} lr {cur:.2e} train {loss.mean():.4e} val {vl:.4e} "
+                        f"({time.perf_counter() - t0:.1f} s)")
+        hist["train_ms"] = train_ms
+        hist["launches"] = launches
+        hist["steps"] = epochs * steps_per_epoch
+        hist["batch"] = b
+        hist["wall_s"] = time.perf_counter() - t0
+        model = Model(net.get_params(), x_mean, x_std, y_mean, y_std, net.shape)
+    finally:
+        if own:
+            net.close()
+    return model, hist
+
+
+def predict(model: Model, X, net: Network | None = None):
+    """Log10 profiles (n x 330) for raw inputs X (n x 12)."""
+    own = net is None
+    net = net or Network(model.shape)
+    try:
+        net.set_params(model.params)
+        y = net.predict((np.asarray(X, dtype=np.float64) - model.x_mean) / model.x_std)
+    finally:
+        if own:
+            net.close()
+    return y.astype(np.float64) * model.y_std + model.y_mean
+
+
+def evaluate(model: Model, n: int, seed: int, *, ref_years=2.0, warm_years=0.25, res_tol=1.0e-6,
+             flux_rtol=0.01, tails=0.05, chunk_days=10.0, log=None):
+    """Warm start against cold spin-up on n drawn columns (layers U{20..40},
+    a fraction `tails` of the inputs outside the training ranges).
+
+    1. reference: cold spin-up for ref_years (its top fluxes F_ref);
+    2. cold: cold spin-up, each column retired once its residual is below
+       res_tol and its top fluxes are within flux_rtol of F_ref;
+    3. warm: surrogate prediction interpolated to each column's layers,
+       then the same spin-up and criterion for at most warm_years; columns
+       still unconverged fall back to a cold spin-up (criterion as in 2).
+    Wall times 2 and 3 (3 including the prediction and the fallback) are the
+    comparison.  Returns a dict of results."""
+    import time
+
+    from . import capi, spinup
+
+    rng = np.random.Generator(np.random.Philox(seed))
+    s = spinup.draw_inputs(rng, n, tails=tails)
+    K = rng.integers(20, 41, size=n)
+    dom = spinup.build(s, K)
+    floor = spinup.flux_floor(s)
+    say = log or (lambda m: None)
+
+    say(f"reference: cold spin-up of {n} columns for {ref_years} y")
+    ref = spinup.spin(dom, ref_years, 1e-300, chunk_days=chunk_days, log=say)
+    F_ref = ref["flux"]
+
+    say("cold: spin-up to the criterion")
+    cold = spinup.spin(dom, ref_years, res_tol, chunk_days=chunk_days, flux_ref=F_ref, flux_rtol=flux_rtol,
+                       floor=floor, log=say)
+
+    say("warm: prediction, spin-up, fallback")
+    X = spinup.as_matrix(s)
+    net = Network(model.shape, max_batch=4096)
+    g = capi.SedimentColumns.from_domain(dom)
+    try:
+        net.set_params(model.params)
+        t0 = time.perf_counter()
+        yn = net.predict((X - model.x_mean) / model.x_std)
+        y = yn.astype(np.float64) * model.y_std + model.y_mean
+        x0 = spinup.from_targets(y, s["depth"], K)
+        t_pred = time.perf_counter() - t0
+        pred_ms, _ = net.last_timing()
+        warm = spinup.spin(dom, warm_years, res_tol, chunk_days=chunk_days, conc=x0, flux_ref=F_ref,
+                           flux_rtol=flux_rtol, floor=floor, columns=g, log=say)
+    finally:
+        net.close()
+        g.close()
+    fb = warm["t_conv"] < 0.0
+    t_fb = 0.0
+    fb_conv = 0
+    if fb.any():
+        sub = dom.subset(np.flatnonzero(fb))
+        say(f"fallback: cold spin-up of {int(fb.sum())} columns")
+        r = spinup.spin(sub, ref_years, res_tol, chunk_days=chunk_days, flux_ref=F_ref[:, fb], flux_rtol=flux_rtol,
+                        floor=floor[fb], log=say)
+        t_fb = r["wall_s"]
+        fb_conv = int((r["t_conv"] >= 0.0).sum())
+    wall_warm = t_pred + warm["wall_s"] + t_fb
+
+    # errors per variable of the warm start itself against the reference state
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(K, out=off[1:])
+    lg = lambda a: np.log10(np.maximum(a, 0.0) + spinup.EPS[:, None])
+    prof_err = np.abs(lg(x0) - lg(ref["state"]))
+    per_var_prof = {VAR: float(np.median(prof_err[v])) for v, VAR in enumerate(_VARS)}
+    sc = np.maximum(np.abs(F_ref), floor[None, :])
+    fl_err = np.abs(warm["flux"] - F_ref) / sc
+    per_var_flux = {VAR: float(np.quantile(fl_err[v], 0.95)) for v, VAR in enumerate(_VARS)}
+    out = {
+        "columns": n, "tails": float(s["tail"].mean()), "ref_years": ref_years, "warm_years": warm_years,
+        "res_tol": res_tol, "flux_rtol": flux_rtol,
+        "wall_reference_s": ref["wall_s"],
+        "wall_cold_s": cold["wall_s"],
+        "cold_unconverged": int(cold["n_unconverged"]),
+        "cold_t_conv_days_median": float(np.median(cold["t_conv"][cold["t_conv"] >= 0]) / 86400.0)
+        if (cold["t_conv"] >= 0).any() else None,
+        "wall_warm_s": wall_warm, "wall_predict_s": t_pred, "predict_kernel_ms": pred_ms,
+        "wall_warm_spinup_s": warm["wall_s"], "wall_fallback_s": t_fb,
+        "warm_t_conv_days_median": float(np.median(warm["t_conv"][~fb]) / 86400.0) if (~fb).any() else None,
+        "fallback_rate": float(fb.mean()), "fallback_rate_tail": float(fb[s["tail"]].mean()) if s["tail"].any()
+        else None, "fallback_converged": fb_conv,
+        "speedup": cold["wall_s"] / wall_warm if wall_warm > 0 else None,
+        "warm_start_log10_error_median_per_var": per_var_prof,
+        "warm_flux_rel_error_p95_per_var": per_var_flux,
+    }
+    return out
+
+
+_VARS = ("ldetC", "sdetC", "ldetN", "sdetN", "detP", "O2", "NO3", "NH4", "ODU", "PO4", "adsP")
+
+
+def _main(argv=None):
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="train the spin-up surrogate / evaluate its warm start")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--data", required=True, help="npz from python -m sedcol.spinup (X, Y)")
+    t.add_argument("--out", required=True)
+    t.add_argument("--epochs", type=int, default=600)
+    t.add_argument("--batch", type=int, default=4096)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--lr-final", type=float, default=2e-5)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--model", required=True)
+    e.add_argument("--n", type=int, default=10000)
+    e.add_argument("--seed", type=int, default=7)
+    e.add_argument("--ref-years", type=float, default=2.0)
+    e.add_argument("--warm-years", type=float, default=0.25)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    if a.cmd == "train":
+        z = np.load(a.data)
+        model, hist = fit(z["X"], z["Y"], epochs=a.epochs, batch=a.batch, lr=a.lr, lr_final=a.lr_final, log=log)
+        model.save(a.out)
+        n_train = int(round(0.9 * z["X"].shape[0]))
+        fl = 3.0 * flops_per_sample() * hist["batch"] * hist["steps"]
+        print(json.dumps({"train_ms": hist["train_ms"], "steps": hist["steps"], "batch": hist["batch"],
+                          "launches": hist["launches"], "final_val_loss": hist["val_loss"][-1],
+                          "n_train": n_train, "train_tflops": fl / (hist["train_ms"] * 1e-3) / 1e12}), flush=True)
+    else:
+        model = Model.load(a.model)
+        r = evaluate(model, a.n, a.seed, ref_years=a.ref_years, warm_years=a.warm_years, log=log)
+        print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    _main()
