@@ -9,7 +9,8 @@
 //              + GELU storing the pre-activation, bias only, GELU'
 //              product, plain fp32)
 //   k_loss     squared error, its gradient in bf16, per-block loss sums
-//   k_colsum   bias gradients (column sums)
+//   k_colsum_part, k_sum_slices  bias gradients (column sums) and the
+//              split-k weight gradients, summed in a fixed order
 //   k_adam     Adam on the fp32 master parameters, writing the bf16 copy
 //   k_gather   minibatch rows of the resident training set, bf16 inputs
 // Layer widths are padded on the device (inputs to 32, outputs to a
@@ -55,12 +56,13 @@ __device__ inline float gelu_grad(float z)
   return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGc * (1.0f + 3.0f * kGa * z * z);
 }
 
-constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int BM = 64, BN = 64, BK = 32;
 enum { EPI_F32 = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_GELU_GRAD = 3 };
 
 struct Epi {
   float *out_f;       // fp32 output (EPI_F32, EPI_BIAS; pre-activation for EPI_BIAS_GELU)
   int ldo;
+  long long zstride;  // EPI_F32 with split k: distance between the slices' outputs
   bf16 *out_h;        // bf16 output (EPI_BIAS_GELU activation, EPI_GELU_GRAD dZ)
   int ldh;
   const float *bias;  // EPI_BIAS, EPI_BIAS_GELU
@@ -68,11 +70,30 @@ struct Epi {
   int ldz;
 };
 
+// four consecutive bf16 (8 bytes) of a row-major array, zero outside
+// [0, lim) of the fastest index; one 8-byte load when all four are inside
+// and the address is aligned
+__device__ inline void load4(const bf16 *__restrict__ p, int i0, int lim, bool row_ok, bf16 (&v)[4])
+{
+  if (row_ok && i0 + 3 < lim && ((reinterpret_cast<uintptr_t>(p + i0) & 7u) == 0)) {
+    const uint2 w = *reinterpret_cast<const uint2 *>(p + i0);
+    v[0] = (bf16)(w.x & 0xffffu);
+    v[1] = (bf16)(w.x >> 16);
+    v[2] = (bf16)(w.y & 0xffffu);
+    v[3] = (bf16)(w.y >> 16);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = (row_ok && i0 + j < lim) ? p[i0 + j] : (bf16)0;
+  }
+}
+
 // C (M x N) = op(A) (M x K) op(B) (K x N); op(A)(m,k) = TA ? A[k lda + m] :
 // A[m lda + k], op(B)(k,n) = TB ? B[n ldb + k] : B[k ldb + n].  Out-of-range
-// rows, columns and k are loaded as zero and not stored.
+// rows, columns and k are loaded as zero and not stored.  Block z of the
+// grid takes k in [z kchunk, (z+1) kchunk) and (EPI_F32) writes its partial
+// product at out_f + z zstride.
 template <bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16 *__restrict__ A, int lda,
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, const bf16 *__restrict__ A, int lda,
                                               const bf16 *__restrict__ B, int ldb, Epi e)
 {
   __shared__ bf16 sA[BM][BK + 4];
@@ -80,38 +101,39 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16 *_
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int kb0 = blockIdx.z * kchunk;
+  const int kend = min(K, kb0 + kchunk);
   f32x16 acc;
 #pragma unroll
   for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
   const int r = lane & 31, q = (lane >> 5) * 4;
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  for (int k0 = kb0; k0 < kend; k0 += BK) {
 #pragma unroll
-    for (int e4 = 0; e4 < 4; ++e4) {
-      const int idx = tid * 4 + e4;  // 0..1023 = 64 x 16 elements of each tile
-      int m, k;
-      if (TA) {
-        m = idx % BM;
-        k = idx / BM;
+    for (int h = 0; h < 2; ++h) {
+      const int idx = tid * 4 + 1024 * h;  // four consecutive elements of the 64 x 32 tile
+      bf16 v[4];
+      if (TA) {  // A stored k-major: m fastest
+        const int m = idx % BM, k = idx / BM;
+        load4(A + (size_t)(k0 + k) * lda, bm + m, M, k0 + k < kend, v);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sA[m + j][k] = v[j];
       } else {
-        m = idx / BK;
-        k = idx % BK;
+        const int m = idx / BK, k = idx % BK;
+        load4(A + (size_t)(bm + m) * lda, k0 + k, kend, bm + m < M, v);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sA[m][k + j] = v[j];
       }
-      const int gm = bm + m, gk = k0 + k;
-      bf16 av = 0;
-      if (gm < M && gk < K) av = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-      sA[m][k] = av;
-      int kb, n;
-      if (TB) {
-        kb = idx % BK;
-        n = idx / BK;
+      if (TB) {  // B stored n-major: k fastest
+        const int kb = idx % BK, n = idx / BK;
+        load4(B + (size_t)(bn + n) * ldb, k0 + kb, kend, bn + n < N, v);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sB[kb + j][n] = v[j];
       } else {
-        kb = idx / BN;
-        n = idx % BN;
+        const int kb = idx / BN, n = idx % BN;
+        load4(B + (size_t)(k0 + kb) * ldb, bn + n, N, k0 + kb < kend, v);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sB[kb][n + j] = v[j];
       }
-      const int gn = bn + n, gkb = k0 + kb;
-      bf16 bv = 0;
-      if (gn < N && gkb < K) bv = TB ? B[(size_t)gn * ldb + gkb] : B[(size_t)gkb * ldb + gn];
-      sB[kb][n] = bv;
     }
     __syncthreads();
 #pragma unroll
@@ -136,7 +158,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16 *_
     if (row >= M) continue;
     const float x = acc[v];
     if constexpr (EPI == EPI_F32) {
-      e.out_f[(size_t)row * e.ldo + col] = x;
+      e.out_f[blockIdx.z * e.zstride + (size_t)row * e.ldo + col] = x;
     } else if constexpr (EPI == EPI_BIAS) {
       e.out_f[(size_t)row * e.ldo + col] = x + e.bias[col];
     } else if constexpr (EPI == EPI_BIAS_GELU) {
@@ -148,6 +170,33 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16 *_
       e.out_h[(size_t)row * e.ldh + col] = to_bf16(x * gelu_grad(z));
     }
   }
+}
+
+// out[i] = sum over s < S of part[s stride + i], in order of s
+__global__ void k_sum_slices(const float *part, int S, long long stride, long long n, float *out)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    float a = 0.0f;
+    for (int sl = 0; sl < S; ++sl) a += part[sl * stride + i];
+    out[i] = a;
+  }
+}
+
+// column sums of dZ (b x n, leading dimension ld) over row chunks of
+// `rows` rows: part[chunk n_p + c]; block = 64 columns x 4 row lanes
+__global__ __launch_bounds__(256) void k_colsum_part(const bf16 *dZ, int b, int n, int ld, int rows, int n_p,
+                                                     float *part)
+{
+  __shared__ float red[4][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), lr = threadIdx.x >> 6;
+  const int r0 = blockIdx.y * rows, r1 = min(b, r0 + rows);
+  float s = 0.0f;
+  if (c < n)
+    for (int i = r0 + lr; i < r1; i += 4) s += from_bf16(dZ[(size_t)i * ld + c]);
+  red[lr][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (lr == 0 && c < n)
+    part[(size_t)blockIdx.y * n_p + c] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 // rows of the fp32 sets (or of X itself when rows is null) into padded
@@ -205,16 +254,6 @@ __global__ __launch_bounds__(256) void k_sum(const float *part, int n, float sca
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = red[0] * scale;
-}
-
-// db[n] = sum over rows of dZ[:, n]
-__global__ void k_colsum(const bf16 *dZ, int b, int n, int ld, float *db)
-{
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  float s = 0.0f;
-  for (int i = 0; i < b; ++i) s += from_bf16(dZ[(size_t)i * ld + c]);
-  db[c] = s;
 }
 
 // Adam on the flat parameter vector; wb receives the bf16 copy
@@ -278,6 +317,8 @@ struct sedsurr_handle {
   float *yh = nullptr, *yb = nullptr;   // max_batch x nout_p
   bf16 *dz0 = nullptr, *dz1 = nullptr;  // max_batch x max(n_hidden, nout_p)
   float *part = nullptr;                // loss partial sums
+  float *kpart = nullptr;               // split-k partial weight gradients
+  float *cpart = nullptr;               // partial column sums
   float *loss = nullptr;                // per-step losses of a train call
   int loss_cap = 0;
   float *X = nullptr, *Y = nullptr;     // training set
@@ -296,13 +337,29 @@ namespace {
 
 constexpr int kLossBlocks = 256;
 
+constexpr int kMaxSplit = 8;     // k slices of a weight-gradient product
+constexpr int kColsumRows = 128;  // rows per partial column sum
+
 template <bool TA, bool TB, int EPI>
 void gemm(sedsurr_handle *h, int M, int N, int K, const bf16 *A, int lda, const bf16 *B, int ldb, ss::Epi e,
-          int *launches)
+          int *launches, int split = 1)
 {
-  dim3 grid((N + ss::BN - 1) / ss::BN, (M + ss::BM - 1) / ss::BM);
-  hipLaunchKernelGGL((ss::k_gemm<TA, TB, EPI>), grid, dim3(256), 0, h->st, M, N, K, A, lda, B, ldb, e);
+  const int kchunk = (K + split - 1) / split;
+  const int kc = (kchunk + ss::BK - 1) / ss::BK * ss::BK;
+  const int nz = (K + kc - 1) / kc;
+  dim3 grid((N + ss::BN - 1) / ss::BN, (M + ss::BM - 1) / ss::BM, nz > 0 ? nz : 1);
+  hipLaunchKernelGGL((ss::k_gemm<TA, TB, EPI>), grid, dim3(256), 0, h->st, M, N, K, kc, A, lda, B, ldb, e);
   ++*launches;
+}
+
+// k slices of the weight-gradient product of a batch of b rows: enough to
+// give the device about 512 blocks, each slice at least 256 rows
+int split_for(int M, int N, int b)
+{
+  const int tiles = ((M + ss::BM - 1) / ss::BM) * ((N + ss::BN - 1) / ss::BN);
+  int s = 1;
+  while (s < kMaxSplit && tiles * s < 512 && b / (2 * s) >= 256) s *= 2;
+  return s;
 }
 
 // width of the activation entering layer l (padded)
@@ -349,14 +406,28 @@ void loss_backward(sedsurr_handle *h, int b, float *loss_dev, int *launches)
   for (int l = L - 1; l >= 0; --l) {
     const bf16 *Hin = l == 0 ? h->xb : h->hs[l - 1];
     const int win = in_width(h, l), wout = h->fop[l];
+    // dW_l = H_in^T dZ_l, split over the batch into slices summed in order
+    const int split = split_for(win, wout, b);
+    const long long welems = (long long)win * wout;
     ss::Epi e{};
-    e.out_f = h->g + h->woff[l];
+    e.out_f = split > 1 ? h->kpart : h->g + h->woff[l];
     e.ldo = wout;
-    // dW_l = H_in^T dZ_l
-    gemm<true, false, ss::EPI_F32>(h, win, wout, b, Hin, win, dz, wout, e, launches);
-    hipLaunchKernelGGL(ss::k_colsum, dim3((wout + 255) / 256), dim3(256), 0, h->st, dz, b, wout, wout,
-                       h->g + h->boff[l]);
-    ++*launches;
+    e.zstride = welems;
+    gemm<true, false, ss::EPI_F32>(h, win, wout, b, Hin, win, dz, wout, e, launches, split);
+    if (split > 1) {
+      const int kc = ((b + split - 1) / split + ss::BK - 1) / ss::BK * ss::BK;
+      const int nz = (b + kc - 1) / kc;
+      hipLaunchKernelGGL(ss::k_sum_slices, dim3(512), dim3(256), 0, h->st, h->kpart, nz, welems, welems,
+                         h->g + h->woff[l]);
+      ++*launches;
+    }
+    // db_l = column sums of dZ_l, by row chunks, summed in order
+    const int chunks = (b + kColsumRows - 1) / kColsumRows;
+    hipLaunchKernelGGL(ss::k_colsum_part, dim3((wout + 63) / 64, chunks), dim3(256), 0, h->st, dz, b, wout, wout,
+                       kColsumRows, wout, h->cpart);
+    hipLaunchKernelGGL(ss::k_sum_slices, dim3((wout + 255) / 256), dim3(256), 0, h->st, h->cpart, chunks,
+                       (long long)wout, (long long)wout, h->g + h->boff[l]);
+    *launches += 2;
     if (l > 0) {
       // dZ_{l-1} = GELU'(Z_{l-1}) * (dZ_l W_l^T)
       ss::Epi d{};
@@ -467,6 +538,10 @@ int sedsurr_create(const sedsurr_shape *shape, sedsurr_handle **out)
   HIPOK_C(hipMalloc(&h->dz0, sizeof(bf16) * B * wmax));
   HIPOK_C(hipMalloc(&h->dz1, sizeof(bf16) * B * wmax));
   HIPOK_C(hipMalloc(&h->part, sizeof(float) * (kLossBlocks + 1)));  // + the loss of sedsurr_loss_grad
+  long long wmax_el = 0;
+  for (int l = 0; l < h->nl; ++l) wmax_el = std::max(wmax_el, (long long)h->fip[l] * h->fop[l]);
+  HIPOK_C(hipMalloc(&h->kpart, sizeof(float) * kMaxSplit * wmax_el));
+  HIPOK_C(hipMalloc(&h->cpart, sizeof(float) * ((B + kColsumRows - 1) / kColsumRows) * wmax));
   HIPOK_C(hipMalloc(&h->xin, sizeof(float) * B * s.n_in));
   HIPOK_C(hipMalloc(&h->yin, sizeof(float) * B * s.n_out));
 #undef HIPOK_C
@@ -479,7 +554,7 @@ int sedsurr_destroy(sedsurr_handle *h)
   if (!h) return SEDCOL_OK;
   if (h->st) (void)hipStreamSynchronize(h->st);
   std::vector<void *> ptrs = {h->w, h->g, h->m, h->v, h->wb, h->xb, h->yh, h->yb, h->dz0, h->dz1,
-                              h->part, h->loss, h->X, h->Y, h->rows, h->xin, h->yin};
+                              h->part, h->loss, h->X, h->Y, h->rows, h->xin, h->yin, h->kpart, h->cpart};
   for (auto *q : h->hs) ptrs.push_back(q);
   for (auto *q : h->zs) ptrs.push_back(q);
   for (void *q : ptrs)

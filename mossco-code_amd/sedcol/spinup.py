@@ -163,6 +163,7 @@ def spin(dom: cases.Domain, years: float, tol: float, chunk_days: float = 10.0, 
     try:
         if conc is not None:
             g.set_state(conc)
+        d0 = g.diagnostics()
         t0 = time.perf_counter()
         t, t_end = 0.0, years * YEAR
         tconv = np.full(dom.ncol, -1.0)
@@ -179,12 +180,15 @@ def spin(dom: cases.Domain, years: float, tol: float, chunk_days: float = 10.0, 
             if nu == 0 or not t < t_end:
                 break
         wall = time.perf_counter() - t0
+        d1 = g.diagnostics()
         x = g.get_state()
         _, fl = g.get_rhs(x)
     finally:
         if columns is None:
             g.close()
-    return {"state": x, "t_conv": tconv, "resid": rs, "n_unconverged": nu, "flux": fl, "wall_s": wall}
+    att = (d1["accepted"] - d0["accepted"]) + (d1["rejected"] - d0["rejected"])
+    return {"state": x, "t_conv": tconv, "resid": rs, "n_unconverged": nu, "flux": fl, "wall_s": wall,
+            "attempts": att}
 
 
 def generate(n: int, seed: int, years: float, tol: float = 1.0e-8, nlayers=K_SURR, tails: float = 0.0,

@@ -60,6 +60,27 @@ def test_loss_and_gradient_match_oracle(net):
     assert cos >= 0.9998
 
 
+def test_split_k_weight_gradient_matches_oracle(gpu_ready):
+    """1000 rows: the weight-gradient products run as two k slices summed
+    in order (and a partial last tile of rows)."""
+    p = _params(7)
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((1000, 12)).astype(np.float32)
+    Y = rng.standard_normal((1000, 330)).astype(np.float32)
+    with surrogate.Network(SHAPE, max_batch=1024) as n:
+        n.set_params(p)
+        lg, gg = n.loss_grad(X, Y)
+    lo, go, _ = M.loss_grad(p.astype(np.float64), X.astype(np.float64), Y.astype(np.float64), SHAPE)
+    assert abs(lg - lo) <= 1e-4 * lo
+    off = 0
+    for (Wo, bo) in M.unflatten(go, *SHAPE):
+        for ref in (Wo.ravel(), bo):
+            err = _rel(gg[off:off + ref.size].astype(np.float64), ref)
+            off += ref.size
+            print(f"  tensor of {ref.size}: rel L2 error {err:.2e}", flush=True)
+            assert err <= 2e-2
+
+
 def test_predict_matches_oracle_in_passes(net):
     p = _params(2)
     net.set_params(p)
