@@ -52,7 +52,7 @@ def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads):
     from sedcol import cases
 
     wet = np.flatnonzero(dom.mask == 0)
-    ncols = min(wet.size, 8 * nthreads)
+    ncols = min(wet.size, 32 * nthreads)
     cols = wet[np.linspace(0, wet.size - 1, ncols).astype(np.int64)]
     sub = dom.subset(cols)
     off = dom.off

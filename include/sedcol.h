@@ -274,6 +274,23 @@ int sedcol_spinup(sedcol_handle *h, double t_start, double chunk, double t_max, 
                   const double *flux_floor, double *t_conv, double *resid,
                   int32_t *n_unconverged);
 
+/* Boundary assembly on the device, the counterpart of get_boundary_conditions
+ * (fabm_sediment_component.F90:1865-2030) for forcing record 0:
+ *   temperature <- temp_surface (ncol; NULL = unchanged);
+ *   for each variable n with present[n] != 0 (the `<name>_at_soil_surface`
+ *   field exists):
+ *     particulate: top flux = -c_surface * w_surface (w: z-velocity, m s-1,
+ *                  negative downward; flux positive downward, :1978-1986);
+ *     dissolved:   bottom-water value = c_surface; top flux =
+ *                  -(c_top - c_surface) / dz_top * (B + D0 + 0.035 T)
+ *                  * phi_top / 86400 / 10000 with bcup_dis = 1 (:2012-2014,
+ *                  from the resident state's top layer), else 0 (:2021).
+ * Variables without a field keep their forcing.  c_surface, w_surface are
+ * (nvar, ncol); flux_out (nvar, ncol, optional) receives the top fluxes.
+ * Not allowed while a forcing series (nrec > 1) is set. */
+int sedcol_assemble_forcing(sedcol_handle *h, const double *temp_surface, const double *c_surface,
+                            const double *w_surface, const int32_t *present, double *flux_out);
+
 /* --- restart ------------------------------------------------------------ */
 
 /* Checkpoint of the device-resident integration state: state, budget

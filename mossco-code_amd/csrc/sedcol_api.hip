@@ -174,9 +174,9 @@ int check_handle(sedcol_handle *h, bool need_state_inputs)
 int alloc_forcing(sedcol_handle *h, int nrec)
 {
   if ((size_t)nrec <= h->forcing_cap) return SEDCOL_OK;
-  hipFree(h->d_temp);
-  hipFree(h->d_cbw);
-  hipFree(h->d_fup);
+  (void)hipFree(h->d_temp);
+  (void)hipFree(h->d_cbw);
+  (void)hipFree(h->d_fup);
   h->d_temp = h->d_cbw = h->d_fup = nullptr;
   h->forcing_cap = 0;
   HIPOK(hipMalloc(&h->d_temp, sizeof(double) * (size_t)nrec * h->ncol));
@@ -531,24 +531,23 @@ int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
 int sedcol_destroy(sedcol_handle *h)
 {
   if (!h) return SEDCOL_OK;
-  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   void *ptrs[] = {h->d_off, h->d_list[0], h->d_list[1], h->d_list[2], h->d_dz, h->d_dzc, h->d_phi, h->d_phiI,
                   h->d_bfac, h->d_biot, h->d_temp, h->d_cbw, h->d_fup, h->d_conc, h->d_cellacc, h->d_colacc,
                   h->d_flux, h->d_diag, h->d_scr, h->d_scr2, h->d_small, h->d_flag, h->d_work,
                   h->d_devview, h->d_params, h->d_wrec, h->d_wrem, h->d_watt, h->d_ctl};
   for (void *q : ptrs)
-    if (q) hipFree(q);
-  if (h->h_flag) hipHostFree(h->h_flag);
-  if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
-  if (h->graph) hipGraphDestroy(h->graph);
+    if (q) (void)hipFree(q);
+  if (h->h_flag) (void)hipHostFree(h->h_flag);
+  drop_graph(h);
   for (int a = 0; a < 2; ++a) {
-    if (h->aux[a]) hipStreamDestroy(h->aux[a]);
-    if (h->evjoin[a]) hipEventDestroy(h->evjoin[a]);
+    if (h->aux[a]) (void)hipStreamDestroy(h->aux[a]);
+    if (h->evjoin[a]) (void)hipEventDestroy(h->evjoin[a]);
   }
-  if (h->evfork) hipEventDestroy(h->evfork);
-  if (h->ev0) hipEventDestroy(h->ev0);
-  if (h->ev1) hipEventDestroy(h->ev1);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->evfork) (void)hipEventDestroy(h->evfork);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return SEDCOL_OK;
 }
@@ -643,6 +642,73 @@ int sedcol_set_forcing_series(sedcol_handle *h, int32_t nrec, double t0, double 
 int sedcol_set_forcing(sedcol_handle *h, const double *temp, const double *c_bw, const double *flux_top)
 {
   return sedcol_set_forcing_series(h, 1, 0.0, INFINITY, temp, c_bw, flux_top, 1);
+}
+
+int sedcol_assemble_forcing(sedcol_handle *h, const double *temp_surface, const double *c_surface,
+                            const double *w_surface, const int32_t *present, double *flux_out)
+{
+  if (!h || !present) return fail(SEDCOL_EINVAL, "null argument");
+  if (!h->have_params) return fail(SEDCOL_EINVAL, "sedcol_set_params has not been called");
+  if (!h->have_geom) return fail(SEDCOL_EINVAL, "sedcol_set_geometry has not been called");
+  if (h->nrec > 1) return fail(SEDCOL_EINVAL, "a forcing series is active; boundary assembly sets one record");
+  unsigned bits = 0;
+  bool need_w = false;
+  for (int n = 0; n < NV; ++n)
+    if (present[n]) {
+      bits |= 1u << n;
+      need_w |= sc::part(n);
+    }
+  if (bits && !c_surface) return fail(SEDCOL_EINVAL, "c_surface is needed for the present fields");
+  if (need_w && !w_surface) return fail(SEDCOL_EINVAL, "w_surface is needed for particulate fields");
+  const size_t nc = (size_t)h->ncol;
+  if (!h->have_forcing) {
+    int rc = alloc_forcing(h, 1);
+    if (rc) return rc;
+    HIPOK(hipMemsetAsync(h->d_temp, 0, sizeof(double) * nc, h->stream));
+    HIPOK(hipMemsetAsync(h->d_cbw, 0, sizeof(double) * NV * nc, h->stream));
+    HIPOK(hipMemsetAsync(h->d_fup, 0, sizeof(double) * NV * nc, h->stream));
+    h->nrec = 1;
+    h->fup_rec_stride = 0;
+    h->t0 = 0.0;
+    h->dt_rec = INFINITY;
+    h->have_forcing = true;
+  }
+  int rc = ensure_scratch(h);  // inputs staged in the NV*ncell scratch (>= (2 NV + 1) ncol when K >= 3)
+  if (rc) return rc;
+  double *d_ts = nullptr, *d_cs = nullptr, *d_ws = nullptr;
+  double *stage = h->d_scr;
+  std::vector<double> tmp;
+  const size_t need = (2 * NV + 1) * nc;
+  double *own = nullptr;
+  if ((size_t)NV * h->ncell < need) {
+    HIPOK(hipMalloc(&own, sizeof(double) * need));
+    stage = own;
+  }
+  hipError_t e = hipSuccess;
+  if (temp_surface) {
+    d_ts = stage;
+    e = hipMemcpyAsync(d_ts, temp_surface, sizeof(double) * nc, hipMemcpyHostToDevice, h->stream);
+  }
+  if (e == hipSuccess && c_surface) {
+    d_cs = stage + nc;
+    e = hipMemcpyAsync(d_cs, c_surface, sizeof(double) * NV * nc, hipMemcpyHostToDevice, h->stream);
+  }
+  if (e == hipSuccess && w_surface) {
+    d_ws = stage + (NV + 1) * nc;
+    e = hipMemcpyAsync(d_ws, w_surface, sizeof(double) * NV * nc, hipMemcpyHostToDevice, h->stream);
+  }
+  const Dev d = h->dev();
+  for (int b = 0; b < 3 && e == hipSuccess; ++b)
+    if (!h->bins[b].empty())
+      sc::launch_assemble(h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, bits, d_ts, d_cs, d_ws, h->d_temp,
+                          h->d_cbw, h->d_fup);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess && flux_out)
+    e = hipMemcpyAsync(flux_out, h->d_fup, sizeof(double) * NV * nc, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (own) (void)hipFree(own);
+  if (e != hipSuccess) return fail(SEDCOL_EHIP, std::string("sedcol_assemble_forcing: ") + hipGetErrorString(e));
+  return SEDCOL_OK;
 }
 
 int sedcol_set_state(sedcol_handle *h, const double *conc)
@@ -895,7 +961,7 @@ int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t rep
       {nullptr, h->d_diag, sizeof(sedcol_coldiag) * nc}, {nullptr, h->d_work, sizeof(int) * nc}};
   auto release = [&]() {
     for (auto &q : snaps)
-      if (q.dst) hipFree(q.dst);
+      if (q.dst) (void)hipFree(q.dst);
   };
   for (auto &q : snaps) {
     if (hipMalloc(&q.dst, q.bytes) != hipSuccess) {
@@ -941,7 +1007,7 @@ int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t rep
             break;
           }
           float ms = 0.f;
-          hipEventElapsedTime(&ms, h->ev0, h->ev1);
+          (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
           if ((rc = read_flag(h, &flag))) break;
           if ((rc = flag_status(flag))) break;
           tmin = std::min(tmin, (double)ms);
@@ -1128,7 +1194,7 @@ int sedcol_get_budget(sedcol_handle *h, double *budget)
   if (e == hipSuccess)
     e = hipMemcpyAsync(budget, d_b, sizeof(double) * SEDCOL_NBUD * h->ncol, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  hipFree(d_b);
+  (void)hipFree(d_b);
   if (e != hipSuccess) return fail(SEDCOL_EHIP, std::string("sedcol_get_budget: ") + hipGetErrorString(e));
   return SEDCOL_OK;
 }

@@ -747,6 +747,51 @@ __global__ __launch_bounds__(BLOCK) void k_residual(Dev d, const int *list, int 
 }
 
 // ---------------------------------------------------------------------------
+// boundary conditions from the pelagic fields at the soil surface
+// (get_boundary_conditions, fabm_sediment_component.F90:1865-2030): one
+// thread per listed column, writing forcing record 0
+// ---------------------------------------------------------------------------
+
+__global__ void k_assemble(Dev d, const int *list, int nlist, double D0, int bcup_dis, unsigned present,
+                           const double *ts, const double *cs, const double *ws, double *temp, double *cbw,
+                           double *fup)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nlist) return;
+  const int c = list[i];
+  const long long top = d.off[c];
+  if (ts) temp[c] = ts[c];  // :1903-1939
+  for (int n = 0; n < NV; ++n) {
+    if (!((present >> n) & 1u)) continue;  // field absent: left as it is (:1963-1967)
+    const long long k = (long long)n * d.ncol + c;
+    if (part(n)) {
+      fup[k] = -(cs[k] * ws[k]);  // :1978-1986, downward positive
+    } else {
+      cbw[k] = cs[k];  // :1989-2000
+      if (bcup_dis == 1) {
+        // :2012-2014, in its operation order
+        double x = (d.conc[n * d.ncell + top] - cbw[k]) / d.dz[top];
+        x = x * ((d.biot[c] + D0) + temp[c] * 0.035);
+        x = x * d.phi[top];
+        x = x / 86400.0;
+        x = x / 10000.0;
+        fup[k] = -x;
+      } else {
+        fup[k] = 0.0;  // :2021
+      }
+    }
+  }
+}
+
+void launch_assemble(hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, unsigned present,
+                     const double *ts, const double *cs, const double *ws, double *temp, double *cbw, double *fup)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_assemble, dim3((n + 255) / 256), dim3(256), 0, st, d, list, n, p.D0, (int)p.bcup_dis, present,
+                     ts, cs, ws, temp, cbw, fup);
+}
+
+// ---------------------------------------------------------------------------
 // launch helpers (host)
 // ---------------------------------------------------------------------------
 

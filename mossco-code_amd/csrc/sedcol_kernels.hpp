@@ -61,6 +61,11 @@ int emp_resident_blocks(int lpc, int blk, const sedcol_params &p);
 void launch_emp_begin(hipStream_t st, const int *list, int n, int r0, int *wrec, double *wrem, long long *watt);
 // v into p[0..n-1] (a kernel, so that it can be captured in a graph)
 void launch_fill_int(hipStream_t st, int *p, int n, int v);
+// boundary conditions of the listed columns from the fields at the soil
+// surface into forcing record 0 (present: bit n set when variable n's
+// field exists)
+void launch_assemble(hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, unsigned present,
+                     const double *ts, const double *cs, const double *ws, double *temp, double *cbw, double *fup);
 void launch_rhs_store(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                       double *r_out, double *s_out);
 void launch_ae_check(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double dt_red, double rcm,

@@ -73,6 +73,15 @@ module sedcol_hip
       type(c_ptr), value :: h
       real(c_double), intent(in) :: temp(*), c_bw(*), flux_top(*)
     end function
+    ! get_boundary_conditions (fabm_sediment_component.F90:1865-2030) on the device
+    integer(c_int) function sedcol_assemble_forcing(h, temp_surface, c_surface, w_surface, present, flux_out) &
+        bind(C, name='sedcol_assemble_forcing')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: temp_surface(*), c_surface(*), w_surface(*)
+      integer(c_int32_t), intent(in) :: present(*)
+      real(c_double), intent(out) :: flux_out(*)
+    end function
     integer(c_int) function sedcol_set_forcing_series(h, nrec, t0, dt_rec, temp, c_bw, flux_top, nrec_flux) &
         bind(C, name='sedcol_set_forcing_series')
       import :: c_int, c_int32_t, c_ptr, c_double

@@ -39,6 +39,7 @@ EXPORTS = (
     "sedcol_residual", "sedcol_get_last_min_dt", "sedcol_synchronize", "sedcol_last_timing",
     "sedcol_set_schedule", "sedcol_get_schedule", "sedcol_set_tuning", "sedcol_get_tuning", "sedcol_autotune",
     "sedcol_checkpoint_size", "sedcol_save_checkpoint", "sedcol_load_checkpoint", "sedcol_spinup",
+    "sedcol_assemble_forcing",
 )
 NBIN = 3
 
@@ -129,6 +130,7 @@ def lib() -> ctypes.CDLL:
             "sedcol_spinup": (ctypes.c_int, [vp, dbl, dbl, dbl, dbl, pd, pd, dbl, pd, pd, pd,
                                              ctypes.POINTER(ctypes.c_int32)]),
             "sedcol_checkpoint_size": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
+            "sedcol_assemble_forcing": (ctypes.c_int, [vp, pd, pd, pd, ctypes.POINTER(ctypes.c_int32), pd]),
             "sedcol_save_checkpoint": (ctypes.c_int, [vp, vp, ctypes.c_int64]),
             "sedcol_load_checkpoint": (ctypes.c_int, [vp, vp, ctypes.c_int64]),
         }
@@ -214,6 +216,20 @@ class SedimentColumns:
         nrec = int(t.shape[0])
         nrf = int(f.shape[0]) if f.ndim == 3 else 1
         check(lib().sedcol_set_forcing_series(self._h, nrec, float(t0), float(dt_rec), _pd(t), _pd(c), _pd(f), nrf))
+
+    def assemble_forcing(self, present, temp_surface=None, c_surface=None, w_surface=None):
+        """Boundary conditions from the fields at the soil surface
+        (get_boundary_conditions); returns the top fluxes (NV, ncol)."""
+        pr = np.ascontiguousarray(np.asarray(present, dtype=np.int32).reshape(NVAR))
+        ts = None if temp_surface is None else _c(temp_surface).reshape(self.ncol)
+        cs = None if c_surface is None else _c(c_surface).reshape(NVAR, self.ncol)
+        ws = None if w_surface is None else _c(w_surface).reshape(NVAR, self.ncol)
+        f = np.empty((NVAR, self.ncol))
+        check(lib().sedcol_assemble_forcing(self._h, _pd(ts) if ts is not None else None,
+                                            _pd(cs) if cs is not None else None,
+                                            _pd(ws) if ws is not None else None,
+                                            pr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _pd(f)))
+        return f
 
     def set_state(self, conc):
         x = _c(conc)

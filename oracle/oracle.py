@@ -85,6 +85,9 @@ def lib():
         L.oc_budget.argtypes = [vp, _pdt, _pdt, _pdt]
         L.oc_inventory.argtypes = [vp, _pdt, _pdt]
         L.oc_residual.argtypes = [vp, vp, _pdt, _pdt, _pdt]
+        L.oc_assemble_forcing.argtypes = [vp, vp, _pdt, _pdt, _pdt, _pdt, ctypes.POINTER(ctypes.c_int32), _pdt,
+                                          _pdt, _pdt]
+        L.oc_assemble_forcing.restype = None
         _LIB = L
     return _LIB
 
@@ -156,6 +159,19 @@ class Oracle:
         t, c, f = (np.ascontiguousarray(a, dtype=np.float64) for a in (temp, cbw, fup))
         self.keep["temp"], self.keep["cbw"], self.keep["fup"] = t, c, f
         self.d.temp, self.d.cbw, self.d.fluxup = _p(t), _p(c), _p(f)
+
+    def assemble_forcing(self, conc, present, temp_surface=None, c_surface=None, w_surface=None):
+        """get_boundary_conditions on the oracle's current forcing (updated in
+        place); returns (temp, cbw, fup) copies."""
+        t, c, f = self.keep["temp"], self.keep["cbw"], self.keep["fup"]
+        pr = np.ascontiguousarray(np.asarray(present, dtype=np.int32))
+        arg = lambda a: None if a is None else _p(np.ascontiguousarray(a, dtype=np.float64))
+        keep = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+                for a in (temp_surface, c_surface, w_surface)]
+        lib().oc_assemble_forcing(ctypes.byref(self.p), ctypes.byref(self.d), _p(np.ascontiguousarray(conc)),
+                                  *(None if a is None else _p(a) for a in keep),
+                                  pr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _p(t), _p(c), _p(f))
+        return t.copy(), c.copy(), f.copy()
 
     def rhs(self, conc):
         x = np.ascontiguousarray(conc, dtype=np.float64)

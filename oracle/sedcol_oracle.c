@@ -952,6 +952,38 @@ int oc_inventory(const oc_domain *d, const double *conc, double *inv)
 }
 
 /* resid[c] = max_{k,n} |rhs_{k,n}| / max(|c_{k,n}|, cfloor[n])  (s-1) */
+void oc_assemble_forcing(const oc_params *p, const oc_domain *d, const double *conc,
+                         const double *temp_surface, const double *c_surface,
+                         const double *w_surface, const int32_t *present,
+                         double *temp, double *cbw, double *fluxup)
+{
+  for (int c = 0; c < d->ncol; ++c) {
+    if (d->mask && d->mask[c]) continue;
+    if (temp_surface) temp[c] = temp_surface[c]; /* :1903-1939 */
+    const long top = d->off[c];
+    for (int n = 0; n < NV; ++n) {
+      if (!present[n]) continue; /* field absent: skipped (:1963-1967) */
+      const long i = (long)n * d->ncol + c;
+      if (oc_part[n]) {
+        fluxup[i] = -(c_surface[i] * w_surface[i]); /* :1978-1986, downward positive */
+      } else {
+        cbw[i] = c_surface[i]; /* :1989-2000 */
+        if (p->bcup_dis == 1) {
+          /* :2012-2014, in its operation order */
+          double x = (conc[(long)n * d->ncell + top] - cbw[i]) / d->dz[top];
+          x = x * ((d->biot[c] + p->D0) + temp[c] * 0.035);
+          x = x * d->phi[top];
+          x = x / 86400.0;
+          x = x / 10000.0;
+          fluxup[i] = -x;
+        } else {
+          fluxup[i] = 0.0; /* :2021 */
+        }
+      }
+    }
+  }
+}
+
 int oc_residual(const oc_params *p, const oc_domain *d, const double *conc,
                 const double *cfloor, double *resid)
 {

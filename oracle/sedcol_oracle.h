@@ -121,6 +121,17 @@ int oc_emp_integrate(const oc_params *p, const oc_domain *d, double *conc,
 int oc_budget(const oc_domain *d, const double *cellacc, const double *colacc,
               double *budget);
 int oc_inventory(const oc_domain *d, const double *conc, double *inv);
+/* Boundary conditions from the pelagic fields at the soil surface, as
+ * get_boundary_conditions (fabm_sediment_component.F90:1865-2030): temp <-
+ * temp_surface; for each variable n with present[n]: particulate ->
+ * fluxup[n] = -c_surface[n] * w_surface[n]; dissolved -> cbw[n] =
+ * c_surface[n] and fluxup[n] = the bcup = 1 flux of :2012-2014 (from the top
+ * layer of conc) or 0.  Land columns are left unchanged.  temp, cbw, fluxup
+ * are updated in place (ncol, NV*ncol, NV*ncol). */
+void oc_assemble_forcing(const oc_params *p, const oc_domain *d, const double *conc,
+                         const double *temp_surface, const double *c_surface,
+                         const double *w_surface, const int32_t *present,
+                         double *temp, double *cbw, double *fluxup);
 int oc_residual(const oc_params *p, const oc_domain *d, const double *conc,
                 const double *cfloor, double *resid);
 

@@ -253,3 +253,32 @@ def test_nan_state_reported(gpu_ready):
             g.run(360.0, 10, RUNGE_KUTTA_4)
         assert e.value.code == capi.SEDCOL_ENAN
         assert g.diagnostics()["status"][0] == 1
+
+
+@pytest.mark.parametrize("bcup_dis", [1, 2])
+def test_assemble_forcing_bitwise(gpu_ready, bcup_dis):
+    """Boundary assembly from the fields at the soil surface on the device
+    against the oracle: top fluxes returned, then the right-hand side under
+    the assembled forcing (temperature and bottom water)."""
+    dom = cases.ragged_columns(ncol=40, seed=8)
+    dom.mask[5] = 1
+    dom.params.bcup_dis = bcup_dis
+    p = dom.params.si()
+    rng = np.random.default_rng(9)
+    x = _perturbed(dom, seed=10)
+    ts = rng.uniform(2.0, 25.0, size=dom.ncol)
+    cs = rng.uniform(0.0, 300.0, size=(11, dom.ncol))
+    ws = -rng.uniform(1e-6, 1e-4, size=(11, dom.ncol))
+    present = np.ones(11, np.int32)
+    present[9] = 0
+    orc = O.Oracle(dom, p)
+    _, _, fo = orc.assemble_forcing(x, present, ts, cs, ws)
+    with capi.SedimentColumns.from_domain(dom) as g:
+        g.set_state(x)
+        fg = g.assemble_forcing(present, ts, cs, ws)
+        rg, flg = g.get_rhs(x)
+    wet = dom.mask == 0
+    assert np.array_equal(fg[:, wet], fo[:, wet])
+    ro, flo = orc.rhs(x)
+    assert np.array_equal(rg, ro)
+    assert np.array_equal(flg, flo)

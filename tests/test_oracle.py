@@ -335,3 +335,47 @@ def test_exp_within_two_ulp():
 def test_rsqrt():
     for x in np.geomspace(1e-30, 1e30, 301):
         assert math.isclose(O.oc_rsqrt(float(x)), 1.0 / math.sqrt(float(x)), rel_tol=1e-15)
+
+
+def test_assemble_forcing_closed_form():
+    """get_boundary_conditions (fabm_sediment_component.F90:1865-2030):
+    particulate flux -c w, solute bottom water copied, bcup = 1 flux from the
+    top layer by the reference formula, absent fields untouched, land
+    columns untouched."""
+    dom = cases.plain_columns(ni=2, nj=2, K=12, seed=3)
+    dom.mask[3] = 1
+    p = dom.params.si()
+    p["bcup_dis"] = 1
+    orc = O.Oracle(dom, p)
+    rng = np.random.default_rng(5)
+    x = dom.conc0.copy()
+    x[:, :] = rng.uniform(0.1, 100.0, size=x.shape)
+    t0, c0, f0 = (a.copy() for a in dom.record(0))
+    ts = rng.uniform(2.0, 25.0, size=4)
+    cs = rng.uniform(0.0, 300.0, size=(11, 4))
+    ws = -rng.uniform(1e-6, 1e-4, size=(11, 4))
+    present = np.ones(11, np.int32)
+    present[7] = 0  # NH4 field absent
+    t, c, f = orc.assemble_forcing(x, present, ts, cs, ws)
+    off = dom.off
+    for col in range(4):
+        if col == 3:
+            assert t[col] == t0[col] and np.array_equal(c[:, col], c0[:, col]) and np.array_equal(f[:, col], f0[:, col])
+            continue
+        assert t[col] == ts[col]
+        top = off[col]
+        for n in range(11):
+            if n == 7:
+                assert c[n, col] == c0[n, col] and f[n, col] == f0[n, col]
+            elif n in (0, 1, 2, 3, 4, 10):
+                assert f[n, col] == -(cs[n, col] * ws[n, col])
+            else:
+                assert c[n, col] == cs[n, col]
+                B, D0 = dom.biot[col], p["D0"]
+                ref = -(x[n, top] - cs[n, col]) / dom.dz[top] * (B + D0 + ts[col] * 0.035) * dom.phi[top] / 86400.0 / 1e4
+                assert f[n, col] == pytest.approx(ref, rel=1e-14, abs=0.0)
+    # bcup_dis = 2: solute fluxes reset to zero
+    p["bcup_dis"] = 2
+    orc = O.Oracle(dom, p)
+    _, _, f2 = orc.assemble_forcing(x, present, ts, cs, ws)
+    assert np.all(f2[[5, 6, 8, 9]][:, :3] == 0.0)
