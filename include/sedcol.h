@@ -110,7 +110,8 @@ typedef struct {
   double emp_rtol, emp_atol;  /* adaptive EMP error tolerances */
   double emp_dt_min, emp_dt_max;
   int32_t bcup_dis;           /* bcup_dissolved_variables: 1 flux, 2 Dirichlet, 3 closed */
-  int32_t bcup_part;          /* bcup_particulate_variables: 1 flux, 3 closed */
+  int32_t bcup_part;          /* bcup_particulate_variables: 1 flux, 2 Dirichlet, 3 closed,
+                                 4 capped flux (sedcol_set_flux_cap) */
   int32_t bcdown;             /* 1, 2 or 3 (reference get_rhs uses 3) */
   int32_t clamp;              /* clamp to the minimum 0 after each fixed step, as
                                  fabm_sediment_component.F90:1726-1731 does */
@@ -273,6 +274,13 @@ int sedcol_spinup(sedcol_handle *h, double t_start, double chunk, double t_max, 
                   const double *cfloor, const double *flux_ref, double flux_rtol,
                   const double *flux_floor, double *t_conv, double *resid,
                   int32_t *n_unconverged);
+
+/* Distributed-POM flux cap per cell (mmol m-2 s-1), the flux_cap of
+ * fabm_sediment_driver.F90:284-291, needed when bcup_part = 4
+ * (distributed_pom_flux, :239-243): the particulate top flux beyond the
+ * first layer's cap passes to the interfaces below (diff3d :791-802).
+ * Cell layout as sedcol_set_geometry. */
+int sedcol_set_flux_cap(sedcol_handle *h, const double *flux_cap);
 
 /* Boundary assembly on the device, the counterpart of get_boundary_conditions
  * (fabm_sediment_component.F90:1865-2030) for forcing record 0:

@@ -47,6 +47,7 @@ struct sedcol_handle {
   int *d_list[3] = {nullptr, nullptr, nullptr};
   double *d_dz = nullptr, *d_dzc = nullptr, *d_phi = nullptr, *d_phiI = nullptr, *d_bfac = nullptr;
   double *d_biot = nullptr;
+  double *d_fcap = nullptr;  // distributed-POM flux cap (bcup_part = 4), ncell
   int nrec = 0;
   long long fup_rec_stride = 0;
   double t0 = 0.0, dt_rec = INFINITY;
@@ -97,6 +98,7 @@ struct sedcol_handle {
     d.phiI = d_phiI;
     d.bfac = d_bfac;
     d.biot = d_biot;
+    d.fcap = d_fcap;
     d.nrec = nrec;
     d.t0 = t0;
     d.dt_rec = dt_rec;
@@ -167,6 +169,8 @@ int check_handle(sedcol_handle *h, bool need_state_inputs)
     if (!h->have_params) return fail(SEDCOL_EINVAL, "sedcol_set_params has not been called");
     if (!h->have_geom) return fail(SEDCOL_EINVAL, "sedcol_set_geometry has not been called");
     if (!h->have_forcing) return fail(SEDCOL_EINVAL, "no forcing set (sedcol_set_forcing)");
+    if (h->p.bcup_part == 4 && !h->d_fcap)
+      return fail(SEDCOL_EINVAL, "bcup_part = 4 needs the flux cap (sedcol_set_flux_cap)");
   }
   return SEDCOL_OK;
 }
@@ -535,7 +539,7 @@ int sedcol_destroy(sedcol_handle *h)
   void *ptrs[] = {h->d_off, h->d_list[0], h->d_list[1], h->d_list[2], h->d_dz, h->d_dzc, h->d_phi, h->d_phiI,
                   h->d_bfac, h->d_biot, h->d_temp, h->d_cbw, h->d_fup, h->d_conc, h->d_cellacc, h->d_colacc,
                   h->d_flux, h->d_diag, h->d_scr, h->d_scr2, h->d_small, h->d_flag, h->d_work,
-                  h->d_devview, h->d_params, h->d_wrec, h->d_wrem, h->d_watt, h->d_ctl};
+                  h->d_devview, h->d_params, h->d_wrec, h->d_wrem, h->d_watt, h->d_ctl, h->d_fcap};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (h->h_flag) (void)hipHostFree(h->h_flag);
@@ -556,10 +560,7 @@ int sedcol_set_params(sedcol_handle *h, const sedcol_params *p)
 {
   if (!h || !p) return fail(SEDCOL_EINVAL, "null argument");
   if (!(p->bcup_dis >= 1 && p->bcup_dis <= 3)) return fail(SEDCOL_EINVAL, "bcup_dis must be 1, 2 or 3");
-  if (p->bcup_part == 4)
-    return fail(SEDCOL_EUNSUPPORTED, "bcup_part = 4 (distributed POM flux) is not implemented on the GPU");
-  if (!(p->bcup_part == 1 || p->bcup_part == 2 || p->bcup_part == 3))
-    return fail(SEDCOL_EINVAL, "bcup_part must be 1, 2 or 3");
+  if (!(p->bcup_part >= 1 && p->bcup_part <= 4)) return fail(SEDCOL_EINVAL, "bcup_part must be 1, 2, 3 or 4");
   if (!(p->bcdown >= 1 && p->bcdown <= 3)) return fail(SEDCOL_EINVAL, "bcdown must be 1, 2 or 3");
   if (!(p->emp_dt_min > 0.0) || !(p->emp_dt_max >= p->emp_dt_min))
     return fail(SEDCOL_EINVAL, "need 0 < emp_dt_min <= emp_dt_max");
@@ -642,6 +643,22 @@ int sedcol_set_forcing_series(sedcol_handle *h, int32_t nrec, double t0, double 
 int sedcol_set_forcing(sedcol_handle *h, const double *temp, const double *c_bw, const double *flux_top)
 {
   return sedcol_set_forcing_series(h, 1, 0.0, INFINITY, temp, c_bw, flux_top, 1);
+}
+
+int sedcol_set_flux_cap(sedcol_handle *h, const double *flux_cap)
+{
+  if (!h || !flux_cap) return fail(SEDCOL_EINVAL, "null argument");
+  std::vector<double> cap;
+  pack_cells(h, flux_cap, cap);
+  for (int c = 0; c < h->ncol; ++c) {
+    if (h->mask[c]) continue;
+    for (int k = h->off[c]; k < h->off[c + 1]; ++k)
+      if (!(cap[k] >= 0.0) || !std::isfinite(cap[k])) return fail(SEDCOL_EDOMAIN, "flux cap must be finite and >= 0");
+  }
+  if (!h->d_fcap) HIPOK(hipMalloc(&h->d_fcap, sizeof(double) * (size_t)h->ncell));
+  HIPOK(hipMemcpyAsync(h->d_fcap, cap.data(), sizeof(double) * cap.size(), hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
 }
 
 int sedcol_assemble_forcing(sedcol_handle *h, const double *temp_surface, const double *c_surface,

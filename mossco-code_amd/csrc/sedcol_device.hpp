@@ -253,6 +253,8 @@ struct Lane {
   double dz, dzc, Dbio, Ddis;
   // fast form: Dbio/dzc, Ddis/dzc (D/dz on the top layer), 1/(phi dz)
   double Gb, Gd, W;
+  // distributed-POM flux cap of the layer (bcup_part = 4; 0 when unset)
+  double cap;
 };
 
 // Interface diffusivities of get_rhs (fabm_sediment_driver.F90:652-653 and
@@ -303,8 +305,29 @@ __device__ __forceinline__ void column_rhs(const sedcol_params &p, const Lane &L
     double F;
     if (ARITH == ARITH_REF) {
       const double D = pt ? L.Dbio : L.Ddis;
-      if (top) F = bc == 1 ? sbc[NV + n] : bc == 2 ? -((D * (x - sbc[n])) / L.dz) : 0.0;
+      if (top) F = (bc == 1 || bc == 4) ? sbc[NV + n] : bc == 2 ? -((D * (x - sbc[n])) / L.dz) : 0.0;
       else F = -((D * (x - xup)) / L.dzc);
+      if (bc == 4) {
+        // diff3d BcUp = 4 (fabm_sediment_driver.F90:791-802): the top flux
+        // beyond the first layer's cap is added to the interfaces below,
+        // each layer's cap taken off in turn, sequentially as written there;
+        // every lane of the group runs the same loop on broadcast caps
+        double rest = sbc[NV + n] - __shfl(L.cap, 0, LPC);
+        bool running = true;  // every check so far passed
+#pragma unroll 1
+        for (int k = 1; k < LPC; ++k) {
+          const double ck = __shfl(L.cap, k, LPC);
+          if (k < L.K) {
+            running = running && rest > 0.0;
+            if (running) {
+              if (L.l == k) F = F + rest;
+              rest = rest - ck;
+            }
+          }
+        }
+        // the loop ran past the last layer: the rest goes to its interface
+        if (running && bottom) F = F + rest;
+      }
       double Fdn = lg_dn(F);
       if (bottom) Fdn = p.bcdown == 2 ? -((D * (0.0 - x)) / L.dz) : 0.0;
       const double VF = pt ? (1.0 - L.phi) : L.phi;

@@ -35,6 +35,7 @@ __device__ __forceinline__ void setup_lane(const Dev &d, int c, int l, Lane &L, 
   L.dzc = (L.active && l > 0) ? d.dzc[cell] : 1.0;
   L.phi = L.active ? d.phi[cell] : 0.5;
   G.phiI = L.active ? d.phiI[cell] : 0.5;
+  L.cap = (L.active && d.fcap) ? d.fcap[cell] : 0.0;
   G.bfac = L.active ? d.bfac[cell] : 0.0;
   G.B = d.biot[c];
 }
@@ -802,7 +803,10 @@ static inline dim3 grid_for(int n)
   return dim3((unsigned)((n + G - 1) / G));
 }
 
-static inline bool use_fast(const sedcol_params &p) { return p.arith == SEDCOL_ARITH_FAST && p.bcdown != 2; }
+static inline bool use_fast(const sedcol_params &p)
+{
+  return p.arith == SEDCOL_ARITH_FAST && p.bcdown != 2 && p.bcup_part != 4;
+}
 
 #define SC_DISPATCH_LPC(lpc, ...)                                \
   switch (lpc) {                                                 \

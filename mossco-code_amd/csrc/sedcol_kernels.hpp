@@ -16,6 +16,7 @@ struct Dev {
   long long ncell;
   const int *off;
   const double *dz, *dzc, *phi, *phiI, *bfac, *biot;
+  const double *fcap;  // ncell or null: distributed-POM flux cap (bcup_part = 4)
   int nrec;
   double t0, dt_rec;
   const double *temp, *cbw, *fup;

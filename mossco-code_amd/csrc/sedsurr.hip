@@ -1,7 +1,7 @@
 // sedsurr.hip -- the spin-up surrogate (include/sedsurr.h) on gfx950.
 //
 // A multilayer perceptron trained and evaluated with hand-written kernels:
-//   k_gemm     bf16 x bf16 -> fp32 on the MFMA units (v_mfma_f32_32x32x8
+//   k_gemm     bf16 x bf16 -> fp32 on the MFMA units (v_mfma_f32_32x32x16
 //              bf16), 64 x 64 block tiles, four waves of 32 x 32, operands
 //              staged through LDS; the three products of training (forward
 //              H W, data gradient dZ W^T, weight gradient H^T dZ) are the
@@ -29,7 +29,7 @@
 namespace ss {
 
 using bf16 = unsigned short;
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // fp32 -> bf16, round to nearest even (NaN kept quiet)
@@ -92,12 +92,18 @@ __device__ inline void load4(const bf16 *__restrict__ p, int i0, int lim, bool r
 // rows, columns and k are loaded as zero and not stored.  Block z of the
 // grid takes k in [z kchunk, (z+1) kchunk) and (EPI_F32) writes its partial
 // product at out_f + z zstride.
+//
+// Both tiles sit in LDS with k contiguous (A as [m][k], B as [n][k]), so a
+// lane's MFMA operand (8 consecutive k of one row or column) is one 16-byte
+// LDS read.  v_mfma_f32_32x32x16_bf16: lane l supplies row (A) / column (B)
+// l % 32 at k = 8 (l / 32) + 0..7.
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, const bf16 *__restrict__ A, int lda,
                                               const bf16 *__restrict__ B, int ldb, Epi e)
 {
-  __shared__ bf16 sA[BM][BK + 4];
-  __shared__ bf16 sB[BK][BN + 4];
+  constexpr int LDK = BK + 8;  // row pitch of the tiles: 80 bytes, 16-byte aligned rows
+  __shared__ __attribute__((aligned(16))) bf16 sA[BM][LDK];
+  __shared__ __attribute__((aligned(16))) bf16 sB[BN][LDK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -106,7 +112,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, c
   f32x16 acc;
 #pragma unroll
   for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
-  const int r = lane & 31, q = (lane >> 5) * 4;
+  const int r = lane & 31, q8 = (lane >> 5) * 8;
   for (int k0 = kb0; k0 < kend; k0 += BK) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -120,31 +126,27 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, c
       } else {
         const int m = idx / BK, k = idx % BK;
         load4(A + (size_t)(bm + m) * lda, k0 + k, kend, bm + m < M, v);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sA[m][k + j] = v[j];
+        *reinterpret_cast<uint2 *>(&sA[m][k]) =
+            make_uint2((unsigned)v[0] | ((unsigned)v[1] << 16), (unsigned)v[2] | ((unsigned)v[3] << 16));
       }
       if (TB) {  // B stored n-major: k fastest
         const int kb = idx % BK, n = idx / BK;
         load4(B + (size_t)(bn + n) * ldb, k0 + kb, kend, bn + n < N, v);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sB[kb + j][n] = v[j];
+        *reinterpret_cast<uint2 *>(&sB[n][kb]) =
+            make_uint2((unsigned)v[0] | ((unsigned)v[1] << 16), (unsigned)v[2] | ((unsigned)v[3] << 16));
       } else {
         const int kb = idx / BN, n = idx % BN;
         load4(B + (size_t)(k0 + kb) * ldb, bn + n, N, k0 + kb < kend, v);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) sB[kb][n + j] = v[j];
+        for (int j = 0; j < 4; ++j) sB[n + j][kb] = v[j];
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 8) {
-      bf16x4 a, b;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        a[j] = (short)sA[wm + r][kk + q + j];
-        b[j] = (short)sB[kk + q + j][wn + r];
-      }
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    for (int kk = 0; kk < BK; kk += 16) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sA[wm + r][kk + q8]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sB[wn + r][kk + q8]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
     __syncthreads();
   }
@@ -152,6 +154,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, c
   // register v holds row 8 (v / 4) + 4 (lane / 32) + v % 4
   const int col = bn + wn + r;
   if (col >= N) return;
+  const int q = (lane >> 5) * 4;
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
     const int row = bm + wm + 8 * (v >> 2) + q + (v & 3);
@@ -365,8 +368,9 @@ int split_for(int M, int N, int b)
 // width of the activation entering layer l (padded)
 inline int in_width(const sedsurr_handle *h, int l) { return h->fip[l]; }
 
-// forward pass of b rows from h->xb; output h->yh (b x nout_p)
-void forward(sedsurr_handle *h, int b, int *launches)
+// forward pass of b rows from h->xb; output h->yh (b x nout_p), or, with
+// out, the n_out real columns only, compact (b x n_out)
+void forward(sedsurr_handle *h, int b, int *launches, float *out = nullptr)
 {
   const int L = h->nl;
   for (int l = 0; l < L; ++l) {
@@ -382,11 +386,11 @@ void forward(sedsurr_handle *h, int b, int *launches)
       gemm<false, false, ss::EPI_BIAS_GELU>(h, b, h->fop[l], h->fip[l], A, lda, h->wb + h->woff[l], h->fop[l], e,
                                             launches);
     } else {
-      e.out_f = h->yh;
-      e.ldo = h->nout_p;
+      e.out_f = out ? out : h->yh;
+      e.ldo = out ? h->s.n_out : h->nout_p;
       e.bias = h->w + h->boff[l];
-      gemm<false, false, ss::EPI_BIAS>(h, b, h->fop[l], h->fip[l], A, lda, h->wb + h->woff[l], h->fop[l], e,
-                                       launches);
+      gemm<false, false, ss::EPI_BIAS>(h, b, out ? h->s.n_out : h->fop[l], h->fip[l], A, lda, h->wb + h->woff[l],
+                                       h->fop[l], e, launches);
     }
   }
 }
@@ -709,10 +713,9 @@ int sedsurr_predict(sedsurr_handle *h, const float *X, int64_t n, float *Y)
     hipLaunchKernelGGL(ss::k_gather, dim3(b), dim3(64), 0, h->st, h->xin, h->s.n_in, (const float *)nullptr, no,
                        (const int *)nullptr, b, h->kin, h->nout_p, h->xb, (float *)nullptr);
     ++launches;
-    forward(h, b, &launches);
+    forward(h, b, &launches, h->yb);  // compact b x n_out output in the target buffer
     HIPOK(hipGetLastError());
-    HIPOK(hipMemcpy2DAsync(Y + r0 * no, sizeof(float) * no, h->yh, sizeof(float) * h->nout_p, sizeof(float) * no,
-                           b, hipMemcpyDeviceToHost, h->st));
+    HIPOK(hipMemcpyAsync(Y + r0 * no, h->yb, sizeof(float) * (size_t)b * no, hipMemcpyDeviceToHost, h->st));
   }
   HIPOK(hipEventRecord(h->e1, h->st));
   HIPOK(hipEventSynchronize(h->e1));

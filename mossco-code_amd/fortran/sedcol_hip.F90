@@ -73,6 +73,12 @@ module sedcol_hip
       type(c_ptr), value :: h
       real(c_double), intent(in) :: temp(*), c_bw(*), flux_top(*)
     end function
+    ! flux_cap of fabm_sediment_driver.F90:284-291 (bcup_part = 4)
+    integer(c_int) function sedcol_set_flux_cap(h, flux_cap) bind(C, name='sedcol_set_flux_cap')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: flux_cap(*)
+    end function
     ! get_boundary_conditions (fabm_sediment_component.F90:1865-2030) on the device
     integer(c_int) function sedcol_assemble_forcing(h, temp_surface, c_surface, w_surface, present, flux_out) &
         bind(C, name='sedcol_assemble_forcing')

@@ -39,7 +39,7 @@ EXPORTS = (
     "sedcol_residual", "sedcol_get_last_min_dt", "sedcol_synchronize", "sedcol_last_timing",
     "sedcol_set_schedule", "sedcol_get_schedule", "sedcol_set_tuning", "sedcol_get_tuning", "sedcol_autotune",
     "sedcol_checkpoint_size", "sedcol_save_checkpoint", "sedcol_load_checkpoint", "sedcol_spinup",
-    "sedcol_assemble_forcing",
+    "sedcol_assemble_forcing", "sedcol_set_flux_cap",
 )
 NBIN = 3
 
@@ -131,6 +131,7 @@ def lib() -> ctypes.CDLL:
                                              ctypes.POINTER(ctypes.c_int32)]),
             "sedcol_checkpoint_size": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
             "sedcol_assemble_forcing": (ctypes.c_int, [vp, pd, pd, pd, ctypes.POINTER(ctypes.c_int32), pd]),
+            "sedcol_set_flux_cap": (ctypes.c_int, [vp, pd]),
             "sedcol_save_checkpoint": (ctypes.c_int, [vp, vp, ctypes.c_int64]),
             "sedcol_load_checkpoint": (ctypes.c_int, [vp, vp, ctypes.c_int64]),
         }
@@ -216,6 +217,10 @@ class SedimentColumns:
         nrec = int(t.shape[0])
         nrf = int(f.shape[0]) if f.ndim == 3 else 1
         check(lib().sedcol_set_forcing_series(self._h, nrec, float(t0), float(dt_rec), _pd(t), _pd(c), _pd(f), nrf))
+
+    def set_flux_cap(self, fcap):
+        """Distributed-POM flux cap per cell (bcup_part = 4)."""
+        check(lib().sedcol_set_flux_cap(self._h, _pd(_c(fcap))))
 
     def assemble_forcing(self, present, temp_surface=None, c_surface=None, w_surface=None):
         """Boundary conditions from the fields at the soil surface

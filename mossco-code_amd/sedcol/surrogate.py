@@ -388,6 +388,10 @@ def _main(argv=None):
     t.add_argument("--batch", type=int, default=4096)
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--lr-final", type=float, default=2e-5)
+    b = sub.add_parser("bench", help="time training steps and prediction on synthetic data of the network's shape")
+    b.add_argument("--steps", type=int, default=200)
+    b.add_argument("--batch", type=int, default=4096)
+    b.add_argument("--rows", type=int, default=1 << 20, help="rows of one prediction call")
     e = sub.add_parser("evaluate")
     e.add_argument("--model", required=True)
     e.add_argument("--n", type=int, default=10000)
@@ -407,6 +411,27 @@ def _main(argv=None):
         print(json.dumps({"train_ms": hist["train_ms"], "steps": hist["steps"], "batch": hist["batch"],
                           "launches": hist["launches"], "final_val_loss": hist["val_loss"][-1],
                           "n_train": n_train, "train_tflops": fl / (hist["train_ms"] * 1e-3) / 1e12}), flush=True)
+    elif a.cmd == "bench":
+        rng = np.random.Generator(np.random.Philox(0))
+        n = 8 * a.batch
+        X = rng.standard_normal((n, SHAPE[0])).astype(np.float32)
+        Y = rng.standard_normal((n, SHAPE[3])).astype(np.float32)
+        with Network(SHAPE, max_batch=a.batch) as net:
+            net.set_params(init_params(rng))
+            net.set_data(X, Y)
+            rows = rng.integers(0, n, size=a.steps * a.batch).astype(np.int32)
+            net.train(rows[: 10 * a.batch], a.batch)  # warm-up
+            net.train(rows, a.batch)
+            ms, launches = net.last_timing()
+            fl = 3.0 * flops_per_sample() * a.batch * a.steps
+            Xp = rng.standard_normal((a.rows, SHAPE[0])).astype(np.float32)
+            net.predict(Xp[: a.batch])
+            net.predict(Xp)
+            pms, plaunch = net.last_timing()
+        print(json.dumps({"train_steps": a.steps, "batch": a.batch, "train_ms": ms, "ms_per_step": ms / a.steps,
+                          "launches_per_step": launches / a.steps, "train_tflops": fl / (ms * 1e-3) / 1e12,
+                          "predict_rows": a.rows, "predict_ms": pms,
+                          "predict_tflops": flops_per_sample() * a.rows / (pms * 1e-3) / 1e12}), flush=True)
     else:
         model = Model.load(a.model)
         r = evaluate(model, a.n, a.seed, years=a.years, warm_days=a.warm_days, log=log)

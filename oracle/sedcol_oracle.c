@@ -301,11 +301,14 @@ static void col_rhs(const oc_params *p, const colview *v, const double *c,
     }
     F1[n] = F[0];
   }
-  /* local rates, then rhs = R + transport (:697-715) */
+  /* local rates, then rhs = R + transport (:697-715); where the fast form
+   * does not apply the whole evaluation is in the reference form */
+  oc_params pk = *p;
+  pk.arith = fast ? p->arith : 0;
   for (int k = 0; k < K; ++k) {
     double cc[NV], R[NV], a, b;
     for (int n = 0; n < NV; ++n) cc[n] = c[(long)n * ks + k];
-    oc_kinetics(p, fT, cc, R, &a, &b);
+    oc_kinetics(&pk, fT, cc, R, &a, &b);
     for (int n = 0; n < NV; ++n) rhs[(long)n * rs + k] = R[n] + rhs[(long)n * rs + k];
     sC[k] = a;
     sN[k] = b;

@@ -233,10 +233,10 @@ def test_errors_are_status_codes(gpu_ready):
         g.set_geometry(dom.dz, dom.dzc, bad, dom.phiI, dom.bfac, dom.biot)
     assert e.value.code == capi.SEDCOL_EDOMAIN
     p = dom.params.si()
-    p["bcup_part"] = 4
+    p["bcup_part"] = 5
     with pytest.raises(capi.SedcolError) as e:
         g.set_params(p)
-    assert e.value.code == capi.SEDCOL_EUNSUPPORTED
+    assert e.value.code == capi.SEDCOL_EINVAL
     with pytest.raises(capi.SedcolError) as e:
         g.run(360.0, 1, RUNGE_KUTTA_4)
     assert e.value.code == capi.SEDCOL_EINVAL
@@ -282,3 +282,62 @@ def test_assemble_forcing_bitwise(gpu_ready, bcup_dis):
     ro, flo = orc.rhs(x)
     assert np.array_equal(rg, ro)
     assert np.array_equal(flg, flo)
+
+
+def _bcup4_domain(seed=12):
+    """Columns of 1..64 layers with flux caps from tiny (the whole top flux
+    cascades past the bottom) to large (no cascade)."""
+    from sedcol import grid
+    rng = np.random.Generator(np.random.Philox(seed))
+    nl = np.array([1, 2, 3, 5, 8, 16, 17, 24, 30, 33, 48, 64] * 2, dtype=np.int32)
+    s = cases.draw_scalars(rng, nl.size)
+    dom = cases.build_columns(nl, 0.2, s)
+    dom.params.bcup_part = 4
+    off = dom.off
+    fcap = np.empty(dom.ncell)
+    for c in range(dom.ncol):
+        sl = slice(off[c], off[c + 1])
+        pom = 10.0 ** rng.uniform(-1.0, 4.5)  # mmol m-3 d-1 scale of pom_flux_max
+        fcap[sl] = grid.flux_cap(dom.phi[sl], dom.dz[sl], pom)
+    return dom, fcap
+
+
+@pytest.mark.parametrize("arith", ARITHS)
+def test_bcup4_rhs_and_emp_bitwise(gpu_ready, arith):
+    """diff3d BcUp = 4 (capped top flux cascading downward,
+    fabm_sediment_driver.F90:791-802) on the GPU against the oracle: one
+    right-hand side, then 6 hours of adaptive EMP (the fast form falls back
+    to the reference form for BcUp = 4)."""
+    dom, fcap = _bcup4_domain()
+    dom.params.arith = arith
+    p = dom.params.si()
+    x = _perturbed(dom, seed=13)
+    orc = O.Oracle(dom, p, fcap=fcap)
+    ro, fo = orc.rhs(x)
+    with capi.SedimentColumns.from_domain(dom) as g:
+        g.set_flux_cap(fcap)
+        rg, fg = g.get_rhs(x)
+        assert np.array_equal(rg, ro)
+        assert np.array_equal(fg, fo)
+        g.integrate(0.0, 6 * 3600.0)
+        xg = g.get_state()
+    xo = dom.conc0.copy()
+    assert orc.emp_integrate(xo, 6 * 3600.0, nthreads=8) == 0
+    assert np.array_equal(xg, xo)
+    assert np.all(xg >= 0.0)
+    # the cascade is active: with the cap removed the right-hand side differs
+    ro1, _ = O.Oracle(dom, dict(p, bcup_part=1)).rhs(x)
+    assert not np.array_equal(ro1, ro)
+
+
+def test_bcup4_needs_flux_cap(gpu_ready):
+    dom, fcap = _bcup4_domain()
+    with capi.SedimentColumns.from_domain(dom) as g:
+        with pytest.raises(capi.SedcolError) as e:
+            g.integrate(0.0, 3600.0)
+        assert e.value.code == capi.SEDCOL_EINVAL
+        bad = fcap.copy()
+        bad[4] = -1.0
+        with pytest.raises(capi.SedcolError) as e:
+            g.set_flux_cap(bad)
+        assert e.value.code == capi.SEDCOL_EDOMAIN

@@ -379,3 +379,57 @@ def test_assemble_forcing_closed_form():
     orc = O.Oracle(dom, p)
     _, _, f2 = orc.assemble_forcing(x, present, ts, cs, ws)
     assert np.all(f2[[5, 6, 8, 9]][:, :3] == 0.0)
+
+
+def test_bcup4_cascade_against_the_reference_loop():
+    """diff3d BcUp = 4 (fabm_sediment_driver.F90:791-802): the oracle's
+    right-hand side minus the BcUp = 1 one equals the divergence of the
+    cascade increments computed here by the reference loop, for columns
+    where the cascade stops inside the column and where it passes the
+    bottom."""
+    from sedcol import grid
+    rng = np.random.Generator(np.random.Philox(2))
+    nl = np.array([1, 3, 9, 20], dtype=np.int32)
+    s = cases.draw_scalars(rng, nl.size)
+    dom = cases.build_columns(nl, 0.2, s)
+    off = dom.off
+    fcap = np.empty(dom.ncell)
+    pom = [1e4, 1.0, 30.0, 5e2]
+    for c in range(dom.ncol):
+        sl = slice(off[c], off[c + 1])
+        fcap[sl] = grid.flux_cap(dom.phi[sl], dom.dz[sl], pom[c])
+    x = dom.conc0.copy()
+    x[0:5] = rng.uniform(10.0, 1e3, size=x[0:5].shape)
+    p4 = dom.params.si()
+    p4["bcup_part"] = 4
+    r4, f4 = O.Oracle(dom, p4, fcap=fcap).rhs(x)
+    r1, f1 = O.Oracle(dom, dict(p4, bcup_part=1), fcap=fcap).rhs(x)
+    fup = dom.record(0)[2]
+    depths = []
+    for c in range(dom.ncol):
+        K = int(nl[c])
+        cap = fcap[off[c]:off[c + 1]]
+        for n in (0, 1, 2, 3, 4, 10):
+            inc = np.zeros(K + 1)  # added to interface k (0 = top)
+            k = 1
+            rest = fup[n, c] - cap[0]
+            while rest > 0 and k < K:
+                inc[k] += rest
+                rest -= cap[k]
+                k += 1
+            if k >= K:
+                inc[K - 1] += rest
+            if n == 0:
+                depths.append(k)
+            phi = dom.phi[off[c]:off[c + 1]]
+            dz = dom.dz[off[c]:off[c + 1]]
+            expect = (inc[:K] - inc[1:K + 1]) / ((1.0 - phi) * dz) * (1.0 - phi) / phi
+            got = r4[n, off[c]:off[c + 1]] - r1[n, off[c]:off[c + 1]]
+            scale = np.abs(r1[n, off[c]:off[c + 1]]).max() + np.abs(expect).max()
+            assert np.allclose(got, expect, rtol=1e-9, atol=1e-12 * scale), (c, n)
+    assert min(depths) < max(nl) and any(d >= K for d, K in zip(depths, nl))
+    # the top flux is the prescribed one, except for a single-layer column,
+    # where the overflow lands on the top interface itself (k >= K at once)
+    assert np.array_equal(f4[:, 1:], f1[:, 1:])
+    assert np.allclose(f4[[0, 1, 2, 3, 4, 10], 0], fup[[0, 1, 2, 3, 4, 10], 0] + (fup[[0, 1, 2, 3, 4, 10], 0] - fcap[0]),
+                       rtol=1e-15)
