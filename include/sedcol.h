@@ -275,6 +275,22 @@ int sedcol_spinup(sedcol_handle *h, double t_start, double chunk, double t_max, 
                   const double *flux_floor, double *t_conv, double *resid,
                   int32_t *n_unconverged);
 
+/* Bioturbation profile 3 (Zhang & Wirtz 2017), fabm_sediment_driver.F90:
+ * 618-645 with the &sed_nml keys of :204-231: on every right-hand side the
+ * bioturbation factor follows the infauna biomass implied by the weighted TOC
+ * profile of the labile and semi-labile detritus carbon (:380-386); the
+ * bioturbation coefficient and its temperature factor are 1 (:622-623).  It
+ * replaces the factor of sedcol_set_geometry until called with NULL, and
+ * runs in the reference arithmetic form.  Detritus carbon must be positive
+ * (the reference divides by the TOC; zero gives NaN, SEDCOL_ENAN). */
+typedef struct {
+  double k_l;         /* bioturb_k_l (0.11) */
+  double L1, L2;      /* bioturb_L1, bioturb_L2 (0.2, 0.6) */
+  double beta, b;     /* bioturb_beta, bioturb_b (0.22, 1.334) */
+  double dry_density; /* bioturb_dry_density, kg m-3 (1000) */
+} sedcol_bioturbation_zw;
+int sedcol_set_bioturbation_zw(sedcol_handle *h, const sedcol_bioturbation_zw *zw);
+
 /* Distributed-POM flux cap per cell (mmol m-2 s-1), the flux_cap of
  * fabm_sediment_driver.F90:284-291, needed when bcup_part = 4
  * (distributed_pom_flux, :239-243): the particulate top flux beyond the

@@ -48,6 +48,12 @@ struct sedcol_handle {
   double *d_dz = nullptr, *d_dzc = nullptr, *d_phi = nullptr, *d_phiI = nullptr, *d_bfac = nullptr;
   double *d_biot = nullptr;
   double *d_fcap = nullptr;  // distributed-POM flux cap (bcup_part = 4), ncell
+  // bioturbation profile 3: parameters, depth terms, the caller's arithmetic
+  // form (profile 3 runs in the reference form)
+  bool zw_on = false;
+  sedcol_bioturbation_zw zwp{};
+  double *d_ze1 = nullptr, *d_ze2 = nullptr, *d_zcum = nullptr;
+  int32_t user_arith = 0;
   int nrec = 0;
   long long fup_rec_stride = 0;
   double t0 = 0.0, dt_rec = INFINITY;
@@ -99,6 +105,17 @@ struct sedcol_handle {
     d.bfac = d_bfac;
     d.biot = d_biot;
     d.fcap = d_fcap;
+    d.zw.on = zw_on ? 1 : 0;
+    d.zw.L1 = zwp.L1;
+    d.zw.L2 = zwp.L2;
+    d.zw.beta = zwp.beta;
+    d.zw.b = zwp.b;
+    // mmol C m-3 -> mg C g-1 dry mass (fabm_sediment_driver.F90:383,386)
+    d.zw.f_labile = 1.0 / 1.2 * 12.01 / zwp.dry_density / 1000.0;
+    d.zw.f_semilabile = 1.0 / 6.0 * 12.01 / zwp.dry_density / 1000.0;
+    d.zw.e1 = d_ze1;
+    d.zw.e2 = d_ze2;
+    d.zw.cum = d_zcum;
     d.nrec = nrec;
     d.t0 = t0;
     d.dt_rec = dt_rec;
@@ -539,7 +556,8 @@ int sedcol_destroy(sedcol_handle *h)
   void *ptrs[] = {h->d_off, h->d_list[0], h->d_list[1], h->d_list[2], h->d_dz, h->d_dzc, h->d_phi, h->d_phiI,
                   h->d_bfac, h->d_biot, h->d_temp, h->d_cbw, h->d_fup, h->d_conc, h->d_cellacc, h->d_colacc,
                   h->d_flux, h->d_diag, h->d_scr, h->d_scr2, h->d_small, h->d_flag, h->d_work,
-                  h->d_devview, h->d_params, h->d_wrec, h->d_wrem, h->d_watt, h->d_ctl, h->d_fcap};
+                  h->d_devview, h->d_params, h->d_wrec, h->d_wrem, h->d_watt, h->d_ctl, h->d_fcap,
+                  h->d_ze1, h->d_ze2, h->d_zcum};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (h->h_flag) (void)hipHostFree(h->h_flag);
@@ -569,8 +587,55 @@ int sedcol_set_params(sedcol_handle *h, const sedcol_params *p)
     return fail(SEDCOL_EINVAL, "arith must be SEDCOL_ARITH_REFERENCE or SEDCOL_ARITH_FAST");
   if (p->reserved != 0) return fail(SEDCOL_EINVAL, "reserved must be 0");
   h->p = *p;
+  h->user_arith = p->arith;
+  if (h->zw_on) h->p.arith = sc::ARITH_EXT;
   h->have_params = true;
   drop_graph(h);  // the kernel instance depends on the arithmetic form
+  return SEDCOL_OK;
+}
+
+namespace {
+int zw_geometry(sedcol_handle *h)
+{
+  const Dev d = h->dev();
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty())
+      sc::launch_zw_geometry(h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->zwp.k_l, h->d_ze1, h->d_ze2,
+                             h->d_zcum);
+  HIPOK(hipGetLastError());
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+}  // namespace
+
+int sedcol_set_bioturbation_zw(sedcol_handle *h, const sedcol_bioturbation_zw *zw)
+{
+  if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  if (!zw) {
+    h->zw_on = false;
+    h->p.arith = h->user_arith;
+    drop_graph(h);
+    return SEDCOL_OK;
+  }
+  if (!h->have_geom) return fail(SEDCOL_EINVAL, "sedcol_set_geometry has not been called");
+  const double v[6] = {zw->k_l, zw->L1, zw->L2, zw->beta, zw->b, zw->dry_density};
+  for (double x : v)
+    if (!std::isfinite(x)) return fail(SEDCOL_EINVAL, "profile 3 parameters must be finite");
+  if (!(zw->dry_density > 0.0) || !(zw->beta > 0.0)) return fail(SEDCOL_EINVAL, "need dry_density > 0, beta > 0");
+  if (!h->d_ze1) {
+    HIPOK(hipMalloc(&h->d_ze1, sizeof(double) * (size_t)h->ncell));
+    HIPOK(hipMalloc(&h->d_ze2, sizeof(double) * (size_t)h->ncell));
+    HIPOK(hipMalloc(&h->d_zcum, sizeof(double) * (size_t)h->ncol));
+  }
+  h->zwp = *zw;
+  h->zw_on = true;
+  int rc = zw_geometry(h);
+  if (rc) {
+    h->zw_on = false;
+    return rc;
+  }
+  h->p.arith = sc::ARITH_EXT;
+  drop_graph(h);
   return SEDCOL_OK;
 }
 
@@ -614,6 +679,7 @@ int sedcol_set_geometry(sedcol_handle *h, const double *dz, const double *dzc, c
   HIPOK(hipMemcpyAsync(h->d_biot, bioturbation, sizeof(double) * h->ncol, hipMemcpyHostToDevice, h->stream));
   HIPOK(hipStreamSynchronize(h->stream));
   h->have_geom = true;
+  if (h->zw_on) return zw_geometry(h);
   return SEDCOL_OK;
 }
 

@@ -28,6 +28,10 @@ namespace sc {
 constexpr int NV = SEDCOL_NVAR;
 constexpr int ARITH_REF = SEDCOL_ARITH_REFERENCE;
 constexpr int ARITH_FAST = SEDCOL_ARITH_FAST;
+// the reference form with the transport options that need per-evaluation
+// work (BcUp = 4, bioturbation profile 3); a kernel instance of its own, so
+// that the plain reference form keeps its register budget
+constexpr int ARITH_EXT = 2;
 
 // particulate flag per state variable (fabm_sediment_driver.F90:658)
 __host__ __device__ constexpr bool part(int n) { return n <= 4 || n == 10; }
@@ -69,6 +73,48 @@ __device__ __forceinline__ double dexp(double x)
   s = fma(s, r, 1.0);
   long long e = (long long)n + 1023;
   return s * __longlong_as_double(e << 52);
+}
+
+// Natural logarithm with the oracle's operation sequence (oc_log): x = m 2^e,
+// m in [sqrt(1/2), sqrt(2)], 2 atanh((m-1)/(m+1)) to s^25, plus e ln2.
+__device__ __forceinline__ double dlog(double x)
+{
+  if (x != x) return x;
+  if (!(x > 0.0)) return x == 0.0 ? -__longlong_as_double(0x7FF0000000000000ll) : __longlong_as_double(0x7FF8000000000000ll);
+  if (x == __longlong_as_double(0x7FF0000000000000ll)) return x;
+  unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  double k = 0.0;
+  if (((b >> 52) & 0x7ffull) == 0) {
+    x = x * 0x1p54;
+    b = (unsigned long long)__double_as_longlong(x);
+    k = -54.0;
+  }
+  long long e = (long long)((b >> 52) & 0x7ffull) - 1023;
+  double m = __longlong_as_double((long long)((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull));
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    e = e + 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double t = 1.0 / 25.0;
+  t = fma(t, s2, 1.0 / 23.0);
+  t = fma(t, s2, 1.0 / 21.0);
+  t = fma(t, s2, 1.0 / 19.0);
+  t = fma(t, s2, 1.0 / 17.0);
+  t = fma(t, s2, 1.0 / 15.0);
+  t = fma(t, s2, 1.0 / 13.0);
+  t = fma(t, s2, 1.0 / 11.0);
+  t = fma(t, s2, 1.0 / 9.0);
+  t = fma(t, s2, 1.0 / 7.0);
+  t = fma(t, s2, 1.0 / 5.0);
+  t = fma(t, s2, 1.0 / 3.0);
+  t = fma(t, s2, 1.0);
+  const double lm = (2.0 * s) * t;
+  k = k + (double)e;
+  const double ln2hi = 6.93147180369123816490e-01;
+  const double ln2lo = 1.90821492927058770002e-10;
+  return fma(k, ln2hi, fma(k, ln2lo, lm));
 }
 
 // 1/sqrt(x), x > 0 finite: exponent-halving guess plus four Newton steps
@@ -118,7 +164,7 @@ __device__ __forceinline__ void kinetics(const sedcol_params &p, double fT, cons
   double Pprod = (kL * dP) * lim;
 
   double oxlim, dnlim, anlim, nitlim, o2odu, odulim;
-  if (ARITH == ARITH_REF) {
+  if (ARITH != ARITH_FAST) {
     oxlim = O2 / (O2 + p.ksO2oxic);
     dnlim = (p.kinO2denit / (O2 + p.kinO2denit)) * (NO3 / (NO3 + p.ksNO3denit));
     anlim = (p.kinO2anox / (O2 + p.kinO2anox)) * (p.kinNO3anox / (NO3 + p.kinNO3anox));
@@ -255,6 +301,18 @@ struct Lane {
   double Gb, Gd, W;
   // distributed-POM flux cap of the layer (bcup_part = 4; 0 when unset)
   double cap;
+  // bioturbation profile 3: exp(100 k_l zc), exp(200 k_l zc), the
+  // column's depth to its last layer, (1/86400/1e4)(1 - phiI) and the
+  // molecular part of the solute diffusivity
+  double ze1, ze2, zcum, zq, Dmol;
+};
+
+// parameters of bioturbation profile 3 (Zhang & Wirtz 2017) as the kernels
+// see them: on = 0 leaves the set_geometry factor in use
+struct ZwDev {
+  int on;
+  double L1, L2, beta, b, f_labile, f_semilabile;
+  const double *e1, *e2, *cum;  // ncell, ncell, ncol
 };
 
 // Interface diffusivities of get_rhs (fabm_sediment_driver.F90:652-653 and
@@ -266,10 +324,16 @@ __device__ __forceinline__ void diffusivities(const sedcol_params &p, double B, 
 {
   L.fT = temp_factor(T);
   const double Dbio = ((((B * L.fT) / 86400.0) / 10000.0) * (1.0 - phiI)) * bfac;
-  const double Ddis = Dbio + (((p.D0 + T * 0.035) * phiI) / 86400.0) / 10000.0;
-  if (ARITH == ARITH_REF) {
+  const double Dmol = (((p.D0 + T * 0.035) * phiI) / 86400.0) / 10000.0;
+  const double Ddis = Dbio + Dmol;
+  if (ARITH != ARITH_FAST) {
     L.Dbio = Dbio;
     L.Ddis = Ddis;
+    if (ARITH == ARITH_EXT) {
+      // profile 3: B = f_T = 1 (fabm_sediment_driver.F90:622-623)
+      L.zq = (((1.0 * 1.0) / 86400.0) / 10000.0) * (1.0 - phiI);
+      L.Dmol = Dmol;
+    }
   } else {
     const double len = L.l == 0 ? L.dz : L.dzc;
     L.Gb = Dbio / len;
@@ -287,7 +351,8 @@ __device__ __forceinline__ void diffusivities(const sedcol_params &p, double B, 
 // The fast form requires bcdown != 2.
 template <int LPC, int ARITH, class CB>
 __device__ __forceinline__ void column_rhs(const sedcol_params &p, const Lane &L, const double *sbc,
-                                           const double (&c)[NV], double &sC, double &sN, CB &&cb)
+                                           const double (&c)[NV], double &sC, double &sN, CB &&cb,
+                                           const ZwDev *zw = nullptr)
 {
   // Callbacks may stage values in LDS for other lanes of the same wave; the
   // wave barriers keep those accesses ordered around the evaluation.
@@ -296,6 +361,22 @@ __device__ __forceinline__ void column_rhs(const sedcol_params &p, const Lane &L
   kinetics<ARITH>(p, L.fT, c, R, sC, sN);
   const bool top = L.l == 0;
   const bool bottom = L.l == L.K - 1;
+  double Dbio = ARITH != ARITH_FAST ? L.Dbio : 0.0;
+  double Ddis = ARITH != ARITH_FAST ? L.Ddis : 0.0;
+  if (ARITH == ARITH_EXT && zw && zw->on) {
+    // bioturbation profile 3 (fabm_sediment_driver.F90:618-645): the factor
+    // follows the infauna biomass implied by the weighted TOC profile of the
+    // two POC classes (:380-386), averaged over the column
+    const double phi = L.phi;
+    double w = 0.0;
+    w = w + ((zw->f_labile * phi) / (1.0 - phi)) * c[0];
+    w = w + ((zw->f_semilabile * phi) / (1.0 - phi)) * c[1];
+    const double avg = lg_sum<LPC>(L.active ? L.dz * w : 0.0) / L.zcum;
+    const double bio = ((w * L.ze1) * avg) / (zw->L1 + zw->L2 * L.ze2);
+    const double bf = L.active ? (zw->beta * dexp(zw->b * dlog(bio))) / w : 0.0;
+    Dbio = L.zq * bf;
+    Ddis = Dbio + L.Dmol;
+  }
 #pragma unroll
   for (int n = 0; n < NV; ++n) {
     const bool pt = part(n);
@@ -303,11 +384,11 @@ __device__ __forceinline__ void column_rhs(const sedcol_params &p, const Lane &L
     const double xup = lg_up(x);
     const int bc = pt ? p.bcup_part : p.bcup_dis;
     double F;
-    if (ARITH == ARITH_REF) {
-      const double D = pt ? L.Dbio : L.Ddis;
+    if (ARITH != ARITH_FAST) {
+      const double D = pt ? Dbio : Ddis;
       if (top) F = (bc == 1 || bc == 4) ? sbc[NV + n] : bc == 2 ? -((D * (x - sbc[n])) / L.dz) : 0.0;
       else F = -((D * (x - xup)) / L.dzc);
-      if (bc == 4) {
+      if (ARITH == ARITH_EXT && bc == 4) {
         // diff3d BcUp = 4 (fabm_sediment_driver.F90:791-802): the top flux
         // beyond the first layer's cap is added to the interfaces below,
         // each layer's cap taken off in turn, sequentially as written there;

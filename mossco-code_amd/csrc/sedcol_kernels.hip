@@ -36,6 +36,9 @@ __device__ __forceinline__ void setup_lane(const Dev &d, int c, int l, Lane &L, 
   L.phi = L.active ? d.phi[cell] : 0.5;
   G.phiI = L.active ? d.phiI[cell] : 0.5;
   L.cap = (L.active && d.fcap) ? d.fcap[cell] : 0.0;
+  L.ze1 = (L.active && d.zw.on) ? d.zw.e1[cell] : 1.0;
+  L.ze2 = (L.active && d.zw.on) ? d.zw.e2[cell] : 1.0;
+  L.zcum = d.zw.on ? d.zw.cum[c] : 1.0;
   G.bfac = L.active ? d.bfac[cell] : 0.0;
   G.B = d.biot[c];
 }
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(Dev d, const int *list, int nlist
   column_rhs<LPC, ARITH>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
     if (L.active) rhs[n * d.ncell + cell] = r;
     if (l == 0 && flux) flux[(long long)n * d.ncol + c] = F;
-  });
+  }, &d.zw);
 }
 
 // ---------------------------------------------------------------------------
@@ -141,7 +144,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
       column_rhs<LPC, ARITH>(p, L, bc, x, sC, sN, [&](int n, double r, double Fv) {
         r0[n] = r;
         if (l == 0) F[0][n] = Fv;
-      });
+      }, &d.zw);
 #pragma unroll
       for (int n = 0; n < NV; ++n) x[n] = x[n] + dt * r0[n];
       if (own) {
@@ -158,27 +161,27 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
         k[n] = 0.5 * r;
         cs[n] = x[n] + (0.5 * dt) * r;
         if (l == 0) F[0][n] = Fv;
-      });
+      }, &d.zw);
       aC = 0.5 * sC;
       aN = 0.5 * sN;
       column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         k[n] = k[n] + r;
         cs[n] = x[n] + (0.5 * dt) * r;
         if (l == 0) F[1][n] = Fv;
-      });
+      }, &d.zw);
       aC = aC + sC;
       aN = aN + sN;
       column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         k[n] = k[n] + r;
         cs[n] = x[n] + dt * r;
         if (l == 0) F[2][n] = Fv;
-      });
+      }, &d.zw);
       aC = aC + sC;
       aN = aN + sN;
       column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         k[n] = k[n] + 0.5 * r;
         if (l == 0) F[3][n] = Fv;
-      });
+      }, &d.zw);
       aC = aC + 0.5 * sC;
       aN = aN + 0.5 * sN;
       const double w = dt * third;
@@ -198,27 +201,27 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
         k1[n] = r;
         cs[n] = x[n] + (third * dt) * r;
         if (l == 0) F[0][n] = Fv;
-      });
+      }, &d.zw);
       s1C = sC;
       s1N = sN;
       column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         k2[n] = r;
         cs[n] = x[n] + dt * (r - third * k1[n]);
         if (l == 0) F[1][n] = Fv;
-      });
+      }, &d.zw);
       s2C = sC;
       s2N = sN;
       column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         cs[n] = x[n] + dt * ((k1[n] - k2[n]) + r);
         k1[n] = (k1[n] + 3.0 * k2[n]) + 3.0 * r;
         if (l == 0) F[2][n] = Fv;
-      });
+      }, &d.zw);
       double aC = (s1C + 3.0 * s2C) + 3.0 * sC;
       double aN = (s1N + 3.0 * s2N) + 3.0 * sN;
       column_rhs<LPC, ARITH>(p, L, bc, cs, sC, sN, [&](int n, double r, double Fv) {
         k1[n] = k1[n] + r;
         if (l == 0) F[3][n] = Fv;
-      });
+      }, &d.zw);
       aC = aC + sC;
       aN = aN + sN;
       const double w = (dt * 1.0) / 8.0;
@@ -235,7 +238,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
       column_rhs<LPC, ARITH>(p, L, bc, x, sC, sN, [&](int n, double r, double Fv) {
         f0[n] = r;
         if (l == 0) F[0][n] = Fv;
-      });
+      }, &d.zw);
       const double p1 = psolve<LPC, false>(x, f0, x, dt, act, zs, dg.newton);
       if (p1 < dg.p_min) dg.p_min = p1;
       if (METHOD == SEDCOL_EMP1) {
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
         column_rhs<LPC, ARITH>(p, L, bc, c1, sC, sN, [&](int n, double r, double Fv) {
           f0[n] = (f0[n] + r) * 0.5;
           if (l == 0) F[1][n] = Fv;
-        });
+        }, &d.zw);
         const double p2 = psolve<LPC, true>(x, f0, c1, dt, act, zs, dg.newton);
         if (p2 < dg.p_min) dg.p_min = p2;
 #pragma unroll
@@ -447,7 +450,7 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
       column_rhs<LPC, ARITH>(p, L, bc, x, s0C, s0N, [&](int n, double rr, double Fv) {
         f0[n] = rr;
         if (l == 0) F[0][n] = Fv;
-      });
+      }, &d.zw);
       const double p1 = psolve<LPC, false>(x, f0, x, h, act, zs, dg.newton);
 #pragma unroll
       for (int n = 0; n < NV; ++n) {
@@ -457,7 +460,7 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
       column_rhs<LPC, ARITH>(p, L, bc, c1, s1C, s1N, [&](int n, double rr, double Fv) {
         f0[n] = (f0[n] + rr) * 0.5;
         if (l == 0) F[1][n] = Fv;
-      });
+      }, &d.zw);
       const double p2 = psolve<LPC, true>(x, f0, c1, h, act, zs, dg.newton);
       // err = max |c2 - c1| / sc: the lane's largest ratio by
       // cross-multiplication over the variables, one division, then the
@@ -587,7 +590,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs_store(Dev d, const int *list, int
   column_rhs<LPC, ARITH>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
     if (L.active) r_out[n * d.ncell + cell] = r;
     if (l == 0) d.flux[(long long)n * d.ncol + c] = F;
-  });
+  }, &d.zw);
   if (L.active) {
     s_out[cell] = sC;
     s_out[d.ncell + cell] = sN;
@@ -742,7 +745,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual(Dev d, const int *list, int 
     const double e = fabs(r) / den;
     if (L.active && (e > m || e != e)) m = e;
     if (l == 0 && flux) flux[(long long)n * d.ncol + c] = F;
-  });
+  }, &d.zw);
   m = lg_max<LPC>(m);
   if (l == 0) resid[c] = m;
 }
@@ -803,9 +806,14 @@ static inline dim3 grid_for(int n)
   return dim3((unsigned)((n + G - 1) / G));
 }
 
-static inline bool use_fast(const sedcol_params &p)
+// kernel instance for the parameters: the extended reference form for
+// BcUp = 4 and bioturbation profile 3 (the host marks the latter with
+// arith = ARITH_EXT), the fast form where it applies, else the reference form
+static inline int arith_form(const sedcol_params &p)
 {
-  return p.arith == SEDCOL_ARITH_FAST && p.bcdown != 2 && p.bcup_part != 4;
+  if (p.arith == ARITH_EXT || p.bcup_part == 4) return ARITH_EXT;
+  if (p.arith == SEDCOL_ARITH_FAST && p.bcdown != 2) return ARITH_FAST;
+  return ARITH_REF;
 }
 
 #define SC_DISPATCH_LPC(lpc, ...)                                \
@@ -815,8 +823,11 @@ static inline bool use_fast(const sedcol_params &p)
     default: { constexpr int LPC = 64; __VA_ARGS__; } break;     \
   }
 #define SC_DISPATCH_ARITH(p, ...)                                          \
-  if (use_fast(p)) { constexpr int AR = ARITH_FAST; __VA_ARGS__; }         \
-  else { constexpr int AR = ARITH_REF; __VA_ARGS__; }
+  switch (arith_form(p)) {                                                 \
+    case ARITH_FAST: { constexpr int AR = ARITH_FAST; __VA_ARGS__; } break; \
+    case ARITH_EXT: { constexpr int AR = ARITH_EXT; __VA_ARGS__; } break;   \
+    default: { constexpr int AR = ARITH_REF; __VA_ARGS__; } break;          \
+  }
 
 void launch_rhs(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                 const double *conc, double *rhs, double *flux)
@@ -905,6 +916,35 @@ __global__ void k_emp_begin(const int *list, int n, int r0, int *wrec, double *w
     wrem[c] = 0.0;
     watt[c] = 0;
   }
+}
+
+// zc from sequential sums of dz (init_grid :161-168), exp(100 k_l zc),
+// exp(200 k_l zc), and the depth to the last layer's top (the cumdepth left by
+// the light loop of get_rhs :593-598); one thread per column, in the
+// oracle's operation order
+__global__ void k_zw_geometry(Dev d, const int *list, int n, double k_l, double *e1, double *e2, double *cum)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = list[i];
+  const int o = d.off[c], K = d.off[c + 1] - o;
+  double zi = 0.0, cd = 0.0;
+  for (int k = 0; k < K; ++k) {
+    const double dz = d.dz[o + k];
+    const double zc = zi + dz * 0.5;
+    e1[o + k] = dexp((zc * 100.0) * k_l);
+    e2[o + k] = dexp((zc * 200.0) * k_l);
+    zi = zi + dz;
+    if (k < K - 1) cd = cd + dz;
+  }
+  cum[c] = cd;
+}
+
+void launch_zw_geometry(hipStream_t st, const Dev &d, const int *list, int n, double k_l, double *e1, double *e2,
+                        double *cum)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zw_geometry, dim3((n + 255) / 256), dim3(256), 0, st, d, list, n, k_l, e1, e2, cum);
 }
 
 __global__ void k_fill_int(int *p, int n, int v)

@@ -17,6 +17,7 @@ struct Dev {
   const int *off;
   const double *dz, *dzc, *phi, *phiI, *bfac, *biot;
   const double *fcap;  // ncell or null: distributed-POM flux cap (bcup_part = 4)
+  ZwDev zw;            // bioturbation profile 3
   int nrec;
   double t0, dt_rec;
   const double *temp, *cbw, *fup;
@@ -60,6 +61,10 @@ int emp_resident_blocks(int lpc, int blk, const sedcol_params &p);
 // start position of every listed column: record r0 - 1, nothing left in it,
 // no attempts
 void launch_emp_begin(hipStream_t st, const int *list, int n, int r0, int *wrec, double *wrem, long long *watt);
+// depth terms of bioturbation profile 3 for the listed columns (zw must
+// point to the device arrays e1, e2 (ncell) and cum (ncol) to fill)
+void launch_zw_geometry(hipStream_t st, const Dev &d, const int *list, int n, double k_l, double *e1, double *e2,
+                        double *cum);
 // v into p[0..n-1] (a kernel, so that it can be captured in a graph)
 void launch_fill_int(hipStream_t st, int *p, int n, int v);
 // boundary conditions of the listed columns from the fields at the soil

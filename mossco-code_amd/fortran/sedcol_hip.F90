@@ -73,6 +73,12 @@ module sedcol_hip
       type(c_ptr), value :: h
       real(c_double), intent(in) :: temp(*), c_bw(*), flux_top(*)
     end function
+    ! bioturbation profile 3 (fabm_sediment_driver.F90:618-645); zw = (k_l, L1, L2, beta, b, dry density)
+    integer(c_int) function sedcol_set_bioturbation_zw(h, zw) bind(C, name='sedcol_set_bioturbation_zw')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: zw(6)
+    end function
     ! flux_cap of fabm_sediment_driver.F90:284-291 (bcup_part = 4)
     integer(c_int) function sedcol_set_flux_cap(h, flux_cap) bind(C, name='sedcol_set_flux_cap')
       import :: c_int, c_ptr, c_double

@@ -39,7 +39,7 @@ EXPORTS = (
     "sedcol_residual", "sedcol_get_last_min_dt", "sedcol_synchronize", "sedcol_last_timing",
     "sedcol_set_schedule", "sedcol_get_schedule", "sedcol_set_tuning", "sedcol_get_tuning", "sedcol_autotune",
     "sedcol_checkpoint_size", "sedcol_save_checkpoint", "sedcol_load_checkpoint", "sedcol_spinup",
-    "sedcol_assemble_forcing", "sedcol_set_flux_cap",
+    "sedcol_assemble_forcing", "sedcol_set_flux_cap", "sedcol_set_bioturbation_zw",
 )
 NBIN = 3
 
@@ -70,6 +70,16 @@ class ColDiag(ctypes.Structure):
 class Schedule(ctypes.Structure):
     _fields_ = [("window", ctypes.c_int32), ("graph_windows", ctypes.c_int32), ("check_every", ctypes.c_int32),
                 ("concurrent_bins", ctypes.c_int32)]
+
+
+class BioturbationZw(ctypes.Structure):
+    _fields_ = [("k_l", ctypes.c_double), ("L1", ctypes.c_double), ("L2", ctypes.c_double),
+                ("beta", ctypes.c_double), ("b", ctypes.c_double), ("dry_density", ctypes.c_double)]
+
+
+#: &sed_nml defaults of bioturbation profile 3 (fabm_sediment_driver.F90:226-231):
+#: k_l, L1, L2, beta, b, dry density
+ZW_DEFAULTS = (0.11, 0.2, 0.6, 0.22, 1.334, 1000.0)
 
 
 class Tuning(ctypes.Structure):
@@ -132,6 +142,7 @@ def lib() -> ctypes.CDLL:
             "sedcol_checkpoint_size": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
             "sedcol_assemble_forcing": (ctypes.c_int, [vp, pd, pd, pd, ctypes.POINTER(ctypes.c_int32), pd]),
             "sedcol_set_flux_cap": (ctypes.c_int, [vp, pd]),
+            "sedcol_set_bioturbation_zw": (ctypes.c_int, [vp, ctypes.POINTER(BioturbationZw)]),
             "sedcol_save_checkpoint": (ctypes.c_int, [vp, vp, ctypes.c_int64]),
             "sedcol_load_checkpoint": (ctypes.c_int, [vp, vp, ctypes.c_int64]),
         }
@@ -217,6 +228,15 @@ class SedimentColumns:
         nrec = int(t.shape[0])
         nrf = int(f.shape[0]) if f.ndim == 3 else 1
         check(lib().sedcol_set_forcing_series(self._h, nrec, float(t0), float(dt_rec), _pd(t), _pd(c), _pd(f), nrf))
+
+    def set_bioturbation_zw(self, zw=ZW_DEFAULTS):
+        """Bioturbation profile 3 (Zhang & Wirtz) with (k_l, L1, L2, beta, b,
+        dry density); None returns to the set_geometry factor."""
+        if zw is None:
+            check(lib().sedcol_set_bioturbation_zw(self._h, None))
+        else:
+            z = BioturbationZw(*(float(v) for v in zw))
+            check(lib().sedcol_set_bioturbation_zw(self._h, ctypes.byref(z)))
 
     def set_flux_cap(self, fcap):
         """Distributed-POM flux cap per cell (bcup_part = 4)."""

@@ -41,10 +41,17 @@ _pdt = ctypes.POINTER(ctypes.c_double)
 _pit = ctypes.POINTER(ctypes.c_int32)
 
 
+class OcZw(ctypes.Structure):
+    _fields_ = [("k_l", ctypes.c_double), ("L1", ctypes.c_double), ("L2", ctypes.c_double),
+                ("beta", ctypes.c_double), ("b", ctypes.c_double), ("f_labile", ctypes.c_double),
+                ("f_semilabile", ctypes.c_double)]
+
+
 class OcDomain(ctypes.Structure):
     _fields_ = [("ncol", ctypes.c_int32), ("ncell", ctypes.c_int32), ("off", _pit), ("dz", _pdt),
                 ("dzc", _pdt), ("phi", _pdt), ("phiI", _pdt), ("bfac", _pdt), ("fcap", _pdt),
-                ("biot", _pdt), ("mask", _pit), ("temp", _pdt), ("cbw", _pdt), ("fluxup", _pdt)]
+                ("biot", _pdt), ("mask", _pit), ("temp", _pdt), ("cbw", _pdt), ("fluxup", _pdt),
+                ("zw", ctypes.POINTER(OcZw))]
 
 
 class OcColDiag(ctypes.Structure):
@@ -73,6 +80,8 @@ def lib():
         vp = ctypes.c_void_p
         L.oc_exp.restype = ctypes.c_double
         L.oc_exp.argtypes = [ctypes.c_double]
+        L.oc_log.restype = ctypes.c_double
+        L.oc_log.argtypes = [ctypes.c_double]
         L.oc_rsqrt.restype = ctypes.c_double
         L.oc_rsqrt.argtypes = [ctypes.c_double]
         L.oc_kinetics.argtypes = [vp, ctypes.c_double, _pdt, _pdt, _pdt, _pdt]
@@ -120,7 +129,9 @@ class Oracle:
     """Oracle view of one Domain (packed layout) and its current forcing
     record.  Keeps the numpy arrays alive for the C structs."""
 
-    def __init__(self, dom, params: dict, record: int = 0, fcap=None):
+    def __init__(self, dom, params: dict, record: int = 0, fcap=None, zw=None):
+        """zw: None, or the bioturbation profile 3 parameters (k_l, L1, L2,
+        beta, b, dry_density), which replace dom.bfac."""
         self.dom = dom
         self.keep = {
             "off": np.ascontiguousarray(dom.off, dtype=np.int32),
@@ -141,6 +152,12 @@ class Oracle:
         for n in ("dz", "dzc", "phi", "phiI", "bfac", "biot"):
             setattr(self.d, n, _p(k[n]))
         self.d.fcap = _p(k["fcap"]) if "fcap" in k else None
+        self.zw = None
+        if zw is not None:
+            k_l, L1, L2, beta, b, rho = zw
+            # mmol C m-3 -> mg C g-1 dry mass (fabm_sediment_driver.F90:383,386)
+            self.zw = OcZw(k_l, L1, L2, beta, b, 1.0 / 1.2 * 12.01 / rho / 1000.0, 1.0 / 6.0 * 12.01 / rho / 1000.0)
+            self.d.zw = ctypes.pointer(self.zw)
         self.set_record(record)
         self.cellacc = np.zeros((5, dom.ncell))
         self.colacc = np.zeros((NV, dom.ncol))
@@ -251,6 +268,10 @@ def kinetics(params: dict, fT: float, c):
 
 def oc_exp(x: float) -> float:
     return lib().oc_exp(float(x))
+
+
+def oc_log(x: float) -> float:
+    return lib().oc_log(float(x))
 
 
 def oc_rsqrt(x: float) -> float:

@@ -68,6 +68,53 @@ double oc_exp(double x)
   return s * two_n;
 }
 
+/* Natural logarithm, the in-house counterpart of oc_exp: x = m 2^e with m in
+ * [sqrt(1/2), sqrt(2)], log m = 2 atanh(s) with s = (m-1)/(m+1) summed to
+ * s^25 (|s| <= 0.1716), plus e ln2 in two parts.  The kernels use the same
+ * operation sequence. */
+double oc_log(double x)
+{
+  if (x != x) return x;
+  if (!(x > 0.0)) return x == 0.0 ? -INFINITY : NAN;
+  if (x == INFINITY) return x;
+  uint64_t b;
+  memcpy(&b, &x, sizeof b);
+  double k = 0.0;
+  if (((b >> 52) & 0x7ff) == 0) { /* subnormal */
+    x = x * 0x1p54;
+    memcpy(&b, &x, sizeof b);
+    k = -54.0;
+  }
+  int64_t e = (int64_t)((b >> 52) & 0x7ff) - 1023;
+  uint64_t mb = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+  double m;
+  memcpy(&m, &mb, sizeof m);
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    e = e + 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double t = 1.0 / 25.0;
+  t = fma(t, s2, 1.0 / 23.0);
+  t = fma(t, s2, 1.0 / 21.0);
+  t = fma(t, s2, 1.0 / 19.0);
+  t = fma(t, s2, 1.0 / 17.0);
+  t = fma(t, s2, 1.0 / 15.0);
+  t = fma(t, s2, 1.0 / 13.0);
+  t = fma(t, s2, 1.0 / 11.0);
+  t = fma(t, s2, 1.0 / 9.0);
+  t = fma(t, s2, 1.0 / 7.0);
+  t = fma(t, s2, 1.0 / 5.0);
+  t = fma(t, s2, 1.0 / 3.0);
+  t = fma(t, s2, 1.0);
+  const double lm = (2.0 * s) * t;
+  k = k + (double)e;
+  const double ln2hi = 6.93147180369123816490e-01;
+  const double ln2lo = 1.90821492927058770002e-10;
+  return fma(k, ln2hi, fma(k, ln2lo, lm));
+}
+
 /* 1/sqrt(x) for finite x > 0: exponent-halving initial guess followed by four
  * Newton steps y <- y (1.5 - (x/2) y^2).  Used only by the step-size
  * controller, where determinism matters more than the last bit. */
@@ -191,6 +238,7 @@ typedef struct {
   const double *dz, *dzc, *phi, *phiI, *bfac, *fcap;
   double B, T;
   double cbw[NV], fup[NV];
+  const oc_zw *zw;
 } colview;
 
 static void make_view(const oc_domain *d, int c, colview *v)
@@ -205,6 +253,7 @@ static void make_view(const oc_domain *d, int c, colview *v)
   v->fcap = d->fcap ? d->fcap + o : NULL;
   v->B = d->biot[c];
   v->T = d->temp[c];
+  v->zw = d->zw;
   for (int n = 0; n < NV; ++n) {
     v->cbw[n] = d->cbw[(size_t)n * d->ncol + c];
     v->fup[n] = d->fluxup[(size_t)n * d->ncol + c];
@@ -244,6 +293,50 @@ static void col_transport_fast(const oc_params *p, const colview *v,
   }
 }
 
+static double tree_sum(double *v, int K);
+
+/* Layer-centre depths zc (sequential sums of dz from the top, as init_grid
+ * :161-168), exp(100 k_l zc), exp(200 k_l zc) and the depth to the last
+ * layer's top (cumdepth after the light loop of :593-598: the dz of layers
+ * 1..K-1), all from the geometry. */
+void oc_zw_geometry(const oc_zw *zw, const double *dz, int K, double *e1, double *e2, double *cum)
+{
+  double zi = 0.0, cd = 0.0;
+  for (int k = 0; k < K; ++k) {
+    const double zc = zi + dz[k] * 0.5;
+    e1[k] = oc_exp((zc * 100.0) * zw->k_l);
+    e2[k] = oc_exp((zc * 200.0) * zw->k_l);
+    zi = zi + dz[k];
+    if (k < K - 1) cd = cd + dz[k];
+  }
+  *cum = cd;
+}
+
+/* bioturbation factor of profile 3 at every layer (fabm_sediment_driver.F90:
+ * 624-642): weighted TOC of the two POC classes, its depth average over the
+ * column (sum of dz * toc, in the pairwise tree, over cumdepth), biomass and
+ * beta biomass^b / toc */
+static void zw_bfac(const colview *v, const double *c, long ks, double *bf)
+{
+  const oc_zw *zw = v->zw;
+  const int K = v->K;
+  double wt[OC_KMAX], t[OC_KMAX], e1[OC_KMAX], e2[OC_KMAX], cum;
+  oc_zw_geometry(zw, v->dz, K, e1, e2, &cum);
+  for (int k = 0; k < K; ++k) {
+    const double phi = v->phi[k];
+    double w = 0.0;
+    w = w + ((zw->f_labile * phi) / (1.0 - phi)) * c[(long)OC_LDETC * ks + k];
+    w = w + ((zw->f_semilabile * phi) / (1.0 - phi)) * c[(long)OC_SDETC * ks + k];
+    wt[k] = w;
+    t[k] = v->dz[k] * w;
+  }
+  const double avg = tree_sum(t, K) / cum;
+  for (int k = 0; k < K; ++k) {
+    const double bio = ((wt[k] * e1[k]) * avg) / (zw->L1 + zw->L2 * e2[k]);
+    bf[k] = (zw->beta * oc_exp(zw->b * oc_log(bio))) / wt[k];
+  }
+}
+
 /* One get_rhs for one column.  c and rhs are [NV][K] with stride ks.
  * F1[n] = top interface flux (intFlux(:,:,1)); sC/sN per-cell sinks.
  * fabm_sediment_driver.F90:575-717 and diff3d :739-825. */
@@ -254,13 +347,25 @@ static void col_rhs(const oc_params *p, const colview *v, const double *c,
   const int K = v->K;
   double Dbio[OC_KMAX], Ddis[OC_KMAX], x[OC_KMAX], F[OC_KMAX + 1];
   double fT = oc_fT(v->T);
+  double bfz[OC_KMAX];
+  const double *bfac = v->bfac;
+  double B = v->B;
+  if (v->zw) {
+    /* profile 3 (:618-645): bioturbation from the infauna biomass that the
+     * TOC profile implies; B and f_T are 1 */
+    zw_bfac(v, c, ks, bfz);
+    bfac = bfz;
+    B = 1.0;
+    fT = 1.0;
+  }
   /* :652-653, then :682-683 for dissolved variables */
   for (int k = 0; k < K; ++k) {
-    Dbio[k] = ((((v->B * fT) / 86400.0) / 10000.0) * (1.0 - v->phiI[k])) * v->bfac[k];
+    Dbio[k] = ((((B * fT) / 86400.0) / 10000.0) * (1.0 - v->phiI[k])) * bfac[k];
     Ddis[k] = Dbio[k] + (((p->D0 + v->T * 0.035) * v->phiI[k]) / 86400.0) / 10000.0;
   }
+  fT = oc_fT(v->T); /* the kinetics keep the temperature factor */
   /* the fast form covers BcUp 1..3 and BcDown 1, 3 */
-  const int fast = p->arith == 1 && p->bcdown != 2 && p->bcup_part != 4;
+  const int fast = p->arith == 1 && p->bcdown != 2 && p->bcup_part != 4 && !v->zw;
   if (fast) col_transport_fast(p, v, c, ks, rhs, rs, F1, Dbio, Ddis);
   for (int n = 0; n < NV && !fast; ++n) {
     const int part = oc_part[n];

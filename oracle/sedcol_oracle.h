@@ -66,6 +66,13 @@ typedef struct {
   int32_t reserved;
 } oc_params;
 
+/* Bioturbation profile 3 (Zhang & Wirtz 2017; fabm_sediment_driver.F90:
+ * 618-645 with the defaults of :226-231 and the POC classes of :380-386). */
+typedef struct {
+  double k_l, L1, L2, beta, b;
+  double f_labile, f_semilabile; /* mmol C m-3 -> mg C g-1 dry mass per class */
+} oc_zw;
+
 typedef struct {
   int32_t ncol;
   int32_t ncell;
@@ -81,6 +88,7 @@ typedef struct {
   const double *temp;   /* ncol, bottom-water temperature (degC) */
   const double *cbw;    /* NV*ncol, bottom-water concentrations */
   const double *fluxup; /* NV*ncol, prescribed top fluxes (positive down) */
+  const oc_zw *zw;      /* NULL, or bioturbation profile 3 (bfac unused) */
 } oc_domain;
 
 /* per-column diagnostics */
@@ -103,6 +111,8 @@ extern "C" {
 #endif
 
 double oc_exp(double x);
+double oc_log(double x);
+void oc_zw_geometry(const oc_zw *zw, const double *dz, int K, double *e1, double *e2, double *cum);
 double oc_rsqrt(double x);
 void oc_kinetics(const oc_params *p, double fT, const double *c, double *R,
                  double *sinkC, double *sinkN);

@@ -341,3 +341,38 @@ def test_bcup4_needs_flux_cap(gpu_ready):
         with pytest.raises(capi.SedcolError) as e:
             g.set_flux_cap(bad)
         assert e.value.code == capi.SEDCOL_EDOMAIN
+
+
+@pytest.mark.parametrize("arith", ARITHS)
+def test_profile3_bioturbation_bitwise(gpu_ready, arith):
+    """Bioturbation profile 3 (Zhang & Wirtz, fabm_sediment_driver.F90:
+    618-645): the factor recomputed from the TOC profile on every right-hand
+    side, on the GPU against the oracle: one right-hand side, then 6 hours of
+    adaptive EMP from a state with detritus (the fast form falls back to the
+    reference form)."""
+    from sedcol import grid
+    rng = np.random.Generator(np.random.Philox(14))
+    nl = np.array([2, 3, 9, 16, 17, 30, 33, 64] * 3, dtype=np.int32)
+    s = cases.draw_scalars(rng, nl.size)
+    dom = cases.build_columns(nl, 0.2, s)
+    dom.params.arith = arith
+    p = dom.params.si()
+    x = _perturbed(dom, seed=15)
+    orc = O.Oracle(dom, p, zw=capi.ZW_DEFAULTS)
+    ro, fo = orc.rhs(x)
+    with capi.SedimentColumns.from_domain(dom) as g:
+        g.set_bioturbation_zw()
+        rg, fg = g.get_rhs(x)
+        assert np.array_equal(rg, ro)
+        assert np.array_equal(fg, fo)
+        g.set_state(x)
+        g.integrate(0.0, 6 * 3600.0)
+        xg = g.get_state()
+        g.set_bioturbation_zw(None)
+        r_off, _ = g.get_rhs(x)
+    xo = x.copy()
+    assert orc.emp_integrate(xo, 6 * 3600.0, nthreads=8) == 0
+    assert np.array_equal(xg, xo)
+    # switched off: the set_geometry factor again
+    assert np.array_equal(r_off, O.Oracle(dom, p).rhs(x)[0])
+    assert not np.array_equal(r_off, ro)

@@ -19,6 +19,7 @@ What pins what:
   * exp/rsqrt -- against libm.
 A planted mistake in any of these formulas fails at least one test here.
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -433,3 +434,56 @@ def test_bcup4_cascade_against_the_reference_loop():
     assert np.array_equal(f4[:, 1:], f1[:, 1:])
     assert np.allclose(f4[[0, 1, 2, 3, 4, 10], 0], fup[[0, 1, 2, 3, 4, 10], 0] + (fup[[0, 1, 2, 3, 4, 10], 0] - fcap[0]),
                        rtol=1e-15)
+
+
+def test_log_within_two_ulp():
+    xs = np.concatenate([np.geomspace(1e-320, 1e300, 4001), np.linspace(0.5, 2.0, 2001),
+                         [1.0, 2.0, 0.7071067811865476, 1.4142135623730951, 5e-324, 1.7976931348623157e308]])
+    for x in xs:
+        v = O.oc_log(float(x))
+        ref = math.log(float(x))
+        assert abs(v - ref) <= 2 * math.ulp(ref) + (1e-300 if ref == 0.0 else 0.0), (x, v, ref)
+    assert O.oc_log(0.0) == -math.inf and math.isnan(O.oc_log(-1.0)) and O.oc_log(math.inf) == math.inf
+
+
+ZW = (0.11, 0.2, 0.6, 0.22, 1.334, 1000.0)  # k_l, L1, L2, beta, b, dry density (fabm_sediment_driver.F90:226-231)
+
+
+def _zw_bfac_numpy(dom, x, zw):
+    """Bioturbation factor of profile 3 (fabm_sediment_driver.F90:618-645),
+    restated with numpy's exp and log."""
+    k_l, L1, L2, beta, b, rho = zw
+    fl, fs = 1.0 / 1.2 * 12.01 / rho / 1000.0, 1.0 / 6.0 * 12.01 / rho / 1000.0
+    out = np.empty(dom.ncell)
+    off = dom.off
+    for c in range(dom.ncol):
+        sl = slice(off[c], off[c + 1])
+        dz, phi = dom.dz[sl], dom.phi[sl]
+        zc = np.cumsum(dz) - dz / 2.0
+        cum = dz[:-1].sum()  # the light loop's last cumdepth: layers 1..K-1
+        wt = fl * phi / (1.0 - phi) * x[0, sl] + fs * phi / (1.0 - phi) * x[1, sl]
+        avg = np.sum(dz * wt) / cum
+        bio = wt * np.exp(zc * 100.0 * k_l) * avg / (L1 + L2 * np.exp(zc * 200.0 * k_l))
+        out[sl] = beta * bio ** b / wt
+    return out
+
+
+def test_profile3_bioturbation_against_numpy():
+    """With profile 3 the oracle's right-hand side equals the constant-profile
+    one whose factor is the numpy restatement, at 15 degC (f_T = 1) and
+    B = 1 (profile 3 sets both to 1)."""
+    dom = cases.plain_columns(ni=2, nj=2, K=20, seed=7)
+    dom.temp[:] = 15.0
+    dom.biot[:] = 1.0
+    p = dom.params.si()
+    rng = np.random.default_rng(3)
+    x = dom.conc0.copy()
+    x[0:5] = rng.uniform(1.0, 5e3, size=x[0:5].shape)
+    r3, f3 = O.Oracle(dom, p, zw=ZW).rhs(x)
+    d2 = dataclasses.replace(dom, bfac=_zw_bfac_numpy(dom, x, ZW))
+    r0, f0 = O.Oracle(d2, p).rhs(x)
+    assert np.allclose(r3, r0, rtol=1e-11, atol=1e-300)
+    assert np.allclose(f3, f0, rtol=1e-11, atol=1e-300)
+    # the factor matters: the set_geometry profile gives another result
+    r1, _ = O.Oracle(dom, p).rhs(x)
+    assert not np.allclose(r1, r3, rtol=1e-6)
