@@ -415,8 +415,8 @@ constexpr char kCkMagic[8] = {'S', 'E', 'D', 'C', 'O', 'L', 'C', 'K'};
 std::vector<std::pair<void *, size_t>> ck_parts(sedcol_handle *h)
 {
   const size_t nc = (size_t)h->ncol, ncell = (size_t)h->ncell;
-  return {{h->d_conc, sizeof(double) * NV * ncell},  {h->d_cellacc, sizeof(double) * 5 * ncell},
-          {h->d_colacc, sizeof(double) * NV * nc},   {h->d_flux, sizeof(double) * NV * nc},
+  return {{h->d_conc, sizeof(double) * NV * ncell},  {h->d_cellacc, sizeof(double) * 10 * ncell},
+          {h->d_colacc, sizeof(double) * 2 * NV * nc}, {h->d_flux, sizeof(double) * NV * nc},
           {h->d_diag, sizeof(sedcol_coldiag) * nc}};
 }
 
@@ -517,8 +517,8 @@ int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
   HIPOK_C(hipMalloc(&h->d_bfac, cb));
   HIPOK_C(hipMalloc(&h->d_biot, sizeof(double) * h->ncol));
   HIPOK_C(hipMalloc(&h->d_conc, NV * cb));
-  HIPOK_C(hipMalloc(&h->d_cellacc, 5 * cb));
-  HIPOK_C(hipMalloc(&h->d_colacc, sizeof(double) * NV * h->ncol));
+  HIPOK_C(hipMalloc(&h->d_cellacc, 10 * cb));
+  HIPOK_C(hipMalloc(&h->d_colacc, sizeof(double) * 2 * NV * h->ncol));
   HIPOK_C(hipMalloc(&h->d_flux, sizeof(double) * NV * h->ncol));
   HIPOK_C(hipMalloc(&h->d_diag, sizeof(sedcol_coldiag) * h->ncol));
   HIPOK_C(hipMalloc(&h->d_small, sizeof(double) * (NV + 1) * h->ncol));
@@ -539,8 +539,8 @@ int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
   HIPOK_C(hipMalloc(&h->d_devview, sizeof(Dev)));
   HIPOK_C(hipMalloc(&h->d_params, sizeof(sedcol_params)));
   HIPOK_C(hipMemset(h->d_conc, 0, NV * cb));
-  HIPOK_C(hipMemset(h->d_cellacc, 0, 5 * cb));
-  HIPOK_C(hipMemset(h->d_colacc, 0, sizeof(double) * NV * h->ncol));
+  HIPOK_C(hipMemset(h->d_cellacc, 0, 10 * cb));
+  HIPOK_C(hipMemset(h->d_colacc, 0, sizeof(double) * 2 * NV * h->ncol));
   HIPOK_C(hipMemset(h->d_flux, 0, sizeof(double) * NV * h->ncol));
 #undef HIPOK_C
   int rc = reset_diag(h);
@@ -1039,8 +1039,8 @@ int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t rep
   const size_t nc = (size_t)h->ncol, ncell = (size_t)h->ncell;
   struct Snap { void *dst; const void *src; size_t bytes; };
   std::vector<Snap> snaps = {
-      {nullptr, h->d_conc, sizeof(double) * NV * ncell}, {nullptr, h->d_cellacc, sizeof(double) * 5 * ncell},
-      {nullptr, h->d_colacc, sizeof(double) * NV * nc},  {nullptr, h->d_flux, sizeof(double) * NV * nc},
+      {nullptr, h->d_conc, sizeof(double) * NV * ncell}, {nullptr, h->d_cellacc, sizeof(double) * 10 * ncell},
+      {nullptr, h->d_colacc, sizeof(double) * 2 * NV * nc}, {nullptr, h->d_flux, sizeof(double) * NV * nc},
       {nullptr, h->d_diag, sizeof(sedcol_coldiag) * nc}, {nullptr, h->d_work, sizeof(int) * nc}};
   auto release = [&]() {
     for (auto &q : snaps)
@@ -1285,8 +1285,8 @@ int sedcol_get_budget(sedcol_handle *h, double *budget)
 int sedcol_reset_budget(sedcol_handle *h)
 {
   if (!h) return fail(SEDCOL_EINVAL, "null handle");
-  HIPOK(hipMemsetAsync(h->d_cellacc, 0, sizeof(double) * 5 * (size_t)h->ncell, h->stream));
-  HIPOK(hipMemsetAsync(h->d_colacc, 0, sizeof(double) * NV * h->ncol, h->stream));
+  HIPOK(hipMemsetAsync(h->d_cellacc, 0, sizeof(double) * 10 * (size_t)h->ncell, h->stream));
+  HIPOK(hipMemsetAsync(h->d_colacc, 0, sizeof(double) * 2 * NV * h->ncol, h->stream));
   HIPOK(hipStreamSynchronize(h->stream));
   return SEDCOL_OK;
 }

@@ -43,6 +43,16 @@ __host__ __device__ constexpr int elem(int n)
 
 __device__ __forceinline__ double nn(double c) { return c < 0.0 ? 0.0 : c; }
 
+// Neumaier-compensated running sum: s + c carries the total of every x added.
+// Same operation order as the oracle's kadd, so sums and compensations match
+// bit for bit.
+__device__ __forceinline__ void kadd(double &s, double &c, double x)
+{
+  const double t = s + x;
+  c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+  s = t;
+}
+
 // exp(x): Cody-Waite reduction with the fdlibm two-part ln2, degree-13 Taylor
 // polynomial in Horner form, 2^n through the exponent field.  Normal range
 // only.  The fixed sequence of IEEE operations makes the result identical on

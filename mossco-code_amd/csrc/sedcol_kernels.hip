@@ -129,10 +129,14 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 #pragma unroll
   for (int n = 0; n < NV; ++n) x[n] = act ? d.conc[n * d.ncell + cell] : 0.0;
   double fint = own ? d.colacc[(long long)l * d.ncol + c] : 0.0;
+  double fintc = own ? d.colacc[(long long)(NV + l) * d.ncol + c] : 0.0;
   double flast = own ? d.flux[(long long)l * d.ncol + c] : 0.0;
-  double acc[5];
+  double acc[5], accc[5];
 #pragma unroll
-  for (int a = 0; a < 5; ++a) acc[a] = act ? d.cellacc[a * d.ncell + cell] : 0.0;
+  for (int a = 0; a < 5; ++a) {
+    acc[a] = act ? d.cellacc[a * d.ncell + cell] : 0.0;
+    accc[a] = act ? d.cellacc[(5 + a) * d.ncell + cell] : 0.0;
+  }
   sedcol_coldiag dg = d.diag[c];
   int64_t zs = 0;
   const double third = 1.0 / 3.0;
@@ -148,11 +152,11 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 #pragma unroll
       for (int n = 0; n < NV; ++n) x[n] = x[n] + dt * r0[n];
       if (own) {
-        fint += dt * F[0][l];
+        kadd(fint, fintc, dt * F[0][l]);
         flast = F[0][l];
       }
-      acc[0] += dt * sC;
-      acc[1] += dt * sN;
+      kadd(acc[0], accc[0], dt * sC);
+      kadd(acc[1], accc[1], dt * sN);
     } else if (METHOD == SEDCOL_RUNGE_KUTTA_4) {
       // solver_library.F90:142-163
       double cs[NV], k[NV];
@@ -188,11 +192,11 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 #pragma unroll
       for (int n = 0; n < NV; ++n) x[n] = x[n] + w * k[n];
       if (own) {
-        fint += w * (((0.5 * F[0][l] + F[1][l]) + F[2][l]) + 0.5 * F[3][l]);
+        kadd(fint, fintc, w * (((0.5 * F[0][l] + F[1][l]) + F[2][l]) + 0.5 * F[3][l]));
         flast = F[3][l];
       }
-      acc[0] += w * aC;
-      acc[1] += w * aN;
+      kadd(acc[0], accc[0], w * aC);
+      kadd(acc[1], accc[1], w * aN);
     } else if (METHOD == SEDCOL_RUNGE_KUTTA_4_38) {
       // solver_library.F90:164-185
       double cs[NV], k1[NV], k2[NV];
@@ -228,11 +232,11 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 #pragma unroll
       for (int n = 0; n < NV; ++n) x[n] = x[n] + w * k1[n];
       if (own) {
-        fint += w * (((F[0][l] + 3.0 * F[1][l]) + 3.0 * F[2][l]) + F[3][l]);
+        kadd(fint, fintc, w * (((F[0][l] + 3.0 * F[1][l]) + 3.0 * F[2][l]) + F[3][l]));
         flast = F[3][l];
       }
-      acc[0] += w * aC;
-      acc[1] += w * aN;
+      kadd(acc[0], accc[0], w * aC);
+      kadd(acc[1], accc[1], w * aN);
     } else {  // EMP-1 / EMP-2 with a fixed step
       double f0[NV];
       column_rhs<LPC, ARITH>(p, L, bc, x, sC, sN, [&](int n, double r, double Fv) {
@@ -245,11 +249,11 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 #pragma unroll
         for (int n = 0; n < NV; ++n) x[n] = x[n] + (dt * p1) * f0[n];
         if (own) {
-          fint += (dt * p1) * F[0][l];
+          kadd(fint, fintc, (dt * p1) * F[0][l]);
           flast = F[0][l];
         }
-        acc[0] += (dt * p1) * sC;
-        acc[1] += (dt * p1) * sN;
+        kadd(acc[0], accc[0], (dt * p1) * sC);
+        kadd(acc[1], accc[1], (dt * p1) * sN);
       } else {
         double c1[NV];
 #pragma unroll
@@ -267,11 +271,11 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 #pragma unroll
         for (int n = 0; n < NV; ++n) x[n] = x[n] + (dt * p2) * f0[n];
         if (own) {
-          fint += (dt * p2) * ((F[0][l] + F[1][l]) * 0.5);
+          kadd(fint, fintc, (dt * p2) * ((F[0][l] + F[1][l]) * 0.5));
           flast = F[1][l];
         }
-        acc[0] += (dt * p2) * ((s0C + sC) * 0.5);
-        acc[1] += (dt * p2) * ((s0N + sN) * 0.5);
+        kadd(acc[0], accc[0], (dt * p2) * ((s0C + sC) * 0.5));
+        kadd(acc[1], accc[1], (dt * p2) * ((s0N + sN) * 0.5));
       }
     }
     // clamp: Patankar rounding guard (EMP), or the callers' minimum clamp
@@ -279,7 +283,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 #pragma unroll
       for (int n = 0; n < NV; ++n)
         if (x[n] < 0.0) {
-          if (elem(n) >= 0) acc[2 + elem(n)] += 0.0 - x[n];
+          if (elem(n) >= 0) kadd(acc[2 + elem(n)], accc[2 + elem(n)], 0.0 - x[n]);
           x[n] = 0.0;
         }
     }
@@ -301,11 +305,15 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
     if (act) d.conc[n * d.ncell + cell] = x[n];
   if (own) {
     d.colacc[(long long)l * d.ncol + c] = fint;
+    d.colacc[(long long)(NV + l) * d.ncol + c] = fintc;
     d.flux[(long long)l * d.ncol + c] = flast;
   }
 #pragma unroll
   for (int a = 0; a < 5; ++a)
-    if (act) d.cellacc[a * d.ncell + cell] = acc[a];
+    if (act) {
+      d.cellacc[a * d.ncell + cell] = acc[a];
+      d.cellacc[(5 + a) * d.ncell + cell] = accc[a];
+    }
   const double zsum = lg_sum<LPC>((double)zs);
   if (l == 0) {
     dg.zero_sink += (int64_t)zsum;
@@ -367,7 +375,13 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
   Geo Gm;
   long long cell = 0;
   double x[NV];
-  double sinkC = 0.0, sinkN = 0.0, fint = 0.0, flast = 0.0;
+  // Per-lane budget sums (with their compensations) and the last top flux:
+  // touched only when an attempt is accepted, so they live in LDS and leave
+  // the registers to the right-hand side.
+  __shared__ double sacc[7][BLK];
+  double &sinkC = sacc[0][threadIdx.x], &sinkN = sacc[1][threadIdx.x], &fint = sacc[2][threadIdx.x];
+  double &sinkCc = sacc[3][threadIdx.x], &sinkNc = sacc[4][threadIdx.x], &fintc = sacc[5][threadIdx.x];
+  double &flast = sacc[6][threadIdx.x];
   sedcol_coldiag dg;
   int64_t zs = 0;
   long long attempts = 0;
@@ -399,7 +413,10 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
       for (int n = 0; n < NV; ++n) x[n] = act ? d.conc[n * d.ncell + cell] : 0.0;
       sinkC = act ? d.cellacc[cell] : 0.0;
       sinkN = act ? d.cellacc[d.ncell + cell] : 0.0;
+      sinkCc = act ? d.cellacc[5 * d.ncell + cell] : 0.0;
+      sinkNc = act ? d.cellacc[6 * d.ncell + cell] : 0.0;
       fint = own ? d.colacc[(long long)l * d.ncol + c] : 0.0;
+      fintc = own ? d.colacc[(long long)(NV + l) * d.ncol + c] : 0.0;
       flast = own ? d.flux[(long long)l * d.ncol + c] : 0.0;
       dg = d.diag[c];
       dt = dg.dt_next;
@@ -504,17 +521,18 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
             if (y < 0.0) {
               // rounding guard; rare, so accumulated straight into the
               // lane's own cell of the budget array
-              if (elem(n) >= 0 && act) d.cellacc[(2 + elem(n)) * d.ncell + cell] += 0.0 - y;
+              if (elem(n) >= 0 && act)
+                kadd(d.cellacc[(2 + elem(n)) * d.ncell + cell], d.cellacc[(7 + elem(n)) * d.ncell + cell], 0.0 - y);
               y = 0.0;
             }
             x[n] = y;
           }
           if (own) {
-            fint += (h * p2) * ((F[0][l] + F[1][l]) * 0.5);
+            kadd(fint, fintc, (h * p2) * ((F[0][l] + F[1][l]) * 0.5));
             flast = F[1][l];
           }
-          sinkC += (h * p2) * ((s0C + s1C) * 0.5);
-          sinkN += (h * p2) * ((s0N + s1N) * 0.5);
+          kadd(sinkC, sinkCc, (h * p2) * ((s0C + s1C) * 0.5));
+          kadd(sinkN, sinkNc, (h * p2) * ((s0N + s1N) * 0.5));
           rem = rem - h;
           dg.accepted += 1;
           if (err > 1.0) dg.forced += 1;
@@ -537,9 +555,12 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
       if (act) {
         d.cellacc[cell] = sinkC;
         d.cellacc[d.ncell + cell] = sinkN;
+        d.cellacc[5 * d.ncell + cell] = sinkCc;
+        d.cellacc[6 * d.ncell + cell] = sinkNc;
       }
       if (own) {
         d.colacc[(long long)l * d.ncol + c] = fint;
+        d.colacc[(long long)(NV + l) * d.ncol + c] = fintc;
         d.flux[(long long)l * d.ncol + c] = flast;
       }
       const double zsum = lg_sum<LPC>((double)zs);
@@ -629,7 +650,9 @@ __global__ __launch_bounds__(BLOCK) void k_ae_commit(Dev d, const int *list, int
   if (slot >= nlist) return;
   const int c = list[slot];
   const int o = d.off[c], K = d.off[c + 1] - o;
-  if (l < NV) d.colacc[(long long)l * d.ncol + c] += dt_red * d.flux[(long long)l * d.ncol + c];
+  if (l < NV)
+    kadd(d.colacc[(long long)l * d.ncol + c], d.colacc[(long long)(NV + l) * d.ncol + c],
+         dt_red * d.flux[(long long)l * d.ncol + c]);
   if (l == 0) {
     sedcol_coldiag dg = d.diag[c];
     dg.accepted += 1;
@@ -641,8 +664,8 @@ __global__ __launch_bounds__(BLOCK) void k_ae_commit(Dev d, const int *list, int
   const long long cell = (long long)o + l;
 #pragma unroll
   for (int n = 0; n < NV; ++n) d.conc[n * d.ncell + cell] = d.conc[n * d.ncell + cell] + dt_red * r_in[n * d.ncell + cell];
-  d.cellacc[cell] += dt_red * s_in[cell];
-  d.cellacc[d.ncell + cell] += dt_red * s_in[d.ncell + cell];
+  kadd(d.cellacc[cell], d.cellacc[5 * d.ncell + cell], dt_red * s_in[cell]);
+  kadd(d.cellacc[d.ncell + cell], d.cellacc[6 * d.ncell + cell], dt_red * s_in[d.ncell + cell]);
 }
 
 // minimum clamp after an ode_solver call (fabm_sediment_component.F90:1726-1731)
@@ -663,7 +686,8 @@ __global__ __launch_bounds__(BLOCK) void k_clamp(Dev d, const int *list, int nli
     const double x = d.conc[n * d.ncell + cell];
     nan |= x != x;
     if (x < 0.0) {
-      if (elem(n) >= 0) d.cellacc[(2 + elem(n)) * d.ncell + cell] += 0.0 - x;
+      if (elem(n) >= 0)
+        kadd(d.cellacc[(2 + elem(n)) * d.ncell + cell], d.cellacc[(7 + elem(n)) * d.ncell + cell], 0.0 - x);
       d.conc[n * d.ncell + cell] = 0.0;
     }
   }
@@ -687,10 +711,10 @@ __global__ __launch_bounds__(BLOCK) void k_budget(Dev d, const int *list, int nl
   const long long cell = (long long)o + l;
   const double w = act ? d.phi[cell] * d.dz[cell] : 0.0;
   double *b = budget + (long long)c * SEDCOL_NBUD;
-  if (l < NV) b[l] = d.colacc[(long long)l * d.ncol + c];
+  if (l < NV) b[l] = d.colacc[(long long)l * d.ncol + c] + d.colacc[(long long)(NV + l) * d.ncol + c];
 #pragma unroll
   for (int a = 0; a < 5; ++a) {
-    const double v = act ? w * d.cellacc[a * d.ncell + cell] : 0.0;
+    const double v = act ? w * (d.cellacc[a * d.ncell + cell] + d.cellacc[(5 + a) * d.ncell + cell]) : 0.0;
     const double s = lg_sum<LPC>(v);
     if (l == 0) b[NV + a] = s;
   }

@@ -23,8 +23,9 @@ struct Dev {
   const double *temp, *cbw, *fup;
   long long fup_rec_stride;  // NV*ncol, or 0 when one flux record serves all
   double *conc;     // NV*ncell
-  double *cellacc;  // 5*ncell: DIC, N2, clamp C, clamp N, clamp P (per m3 pore water)
-  double *colacc;   // NV*ncol: time-integrated top flux
+  // Budget sums, each followed by its Neumaier compensation (total = sum + comp):
+  double *cellacc;  // 10*ncell: DIC, N2, clamp C, clamp N, clamp P (per m3 pore water), then comps
+  double *colacc;   // 2*NV*ncol: time-integrated top flux per variable, then comps
   double *flux;     // NV*ncol: top flux of the last right-hand side
   sedcol_coldiag *diag;
   // adaptive integration position of each column between windows

@@ -40,13 +40,15 @@ EMP_ATTEMPT_PER_CELL = 2 * RHS_PER_CELL + 2 * NV * 2 + NV * 2 + NV * 2 + NV * 2 
 NEWTON_PER_CELL = NV * 6 + 4
 RK4_STEP_PER_CELL = 4 * RHS_PER_CELL + 3 * NV * 2 + NV * 4 + 12
 
-#: HBM bytes a persistent launch moves per cell and per column, independent
-#: of the number of steps inside it: state in+out (2*11*8), cell
-#: accumulators in+out (2*5*8), geometry (dz, dzc, phi, phiI, bfac: 5*8);
-#: per column: offsets, bioturbation, flux/colacc in+out, diagnostics, and
-#: per forcing record temperature + 2*11 forcing values.
-BYTES_PER_CELL_LAUNCH = 2 * NV * 8 + 2 * 5 * 8 + 5 * 8
-BYTES_PER_COLUMN_LAUNCH = 8 + 8 + 2 * 2 * NV * 8 + 2 * 88
+#: HBM bytes a persistent adaptive launch moves per cell and per column,
+#: independent of the number of steps inside it: state in+out (2*11*8), the
+#: DIC and N2 sink sums with their compensations in+out (2*4*8), geometry
+#: (dz, dzc, phi, phiI, bfac: 5*8); per column: offsets, bioturbation, top
+#: flux, its time integral and that integral's compensation in+out
+#: (2*3*11*8), diagnostics in+out, and per forcing record temperature +
+#: 2*11 forcing values.
+BYTES_PER_CELL_LAUNCH = 2 * NV * 8 + 2 * 4 * 8 + 5 * 8
+BYTES_PER_COLUMN_LAUNCH = 8 + 8 + 2 * 3 * NV * 8 + 2 * 88
 BYTES_PER_COLUMN_RECORD = 8 + 2 * NV * 8
 
 

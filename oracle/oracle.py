@@ -159,8 +159,9 @@ class Oracle:
             self.zw = OcZw(k_l, L1, L2, beta, b, 1.0 / 1.2 * 12.01 / rho / 1000.0, 1.0 / 6.0 * 12.01 / rho / 1000.0)
             self.d.zw = ctypes.pointer(self.zw)
         self.set_record(record)
-        self.cellacc = np.zeros((5, dom.ncell))
-        self.colacc = np.zeros((NV, dom.ncol))
+        # running sums followed by their compensations (sum + comp = total)
+        self.cellacc = np.zeros((10, dom.ncell))
+        self.colacc = np.zeros((2 * NV, dom.ncol))
         self.flux = np.zeros((NV, dom.ncol))
         self.diag = np.zeros(dom.ncol, dtype=DIAG_DTYPE)
         self.diag["dt_minacc"] = np.inf

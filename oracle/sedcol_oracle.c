@@ -543,6 +543,18 @@ static double col_psolve(const double *c, long ks, const double *g, long gs,
 /* accumulators                                                              */
 /* cellacc[a*ncell + cell]: a = 0 sinkC, 1 sinkN, 2 clampC, 3 clampN, 4 clampP */
 /* colacc[n*ncol + c]: time-integrated top flux per variable                  */
+/* Both are running sums with Neumaier compensation: the compensation of     */
+/* cellacc[a*ncell + cell] is cellacc[(5+a)*ncell + cell], that of            */
+/* colacc[n*ncol + c] is colacc[(NV+n)*ncol + c]; a total is sum + comp.      */
+
+static void kadd(double *sum, double *comp, double x)
+{
+  const double t = *sum + x;
+  *comp += fabs(*sum) >= fabs(x) ? (*sum - t) + x : (x - t) + *sum;
+  *sum = t;
+}
+#define ACC_CELL(acc, ncell, a, k, x) kadd(&(acc)[(long)(a) * (ncell) + (k)], &(acc)[(long)(5 + (a)) * (ncell) + (k)], (x))
+#define ACC_COL(acc, ncol, n, x) kadd(&(acc)[(long)(n) * (ncol)], &(acc)[(long)(NV + (n)) * (ncol)], (x))
 /* ------------------------------------------------------------------------ */
 
 static void clamp_col(const colview *v, double *c, long ks, double *cellacc,
@@ -553,7 +565,7 @@ static void clamp_col(const colview *v, double *c, long ks, double *cellacc,
       double *x = c + (long)n * ks + k;
       if (*x < 0.0) {
         double add = 0.0 - *x;
-        if (cellacc && oc_elem[n] >= 0) cellacc[(2 + oc_elem[n]) * ncell + k] += add;
+        if (cellacc && oc_elem[n] >= 0) ACC_CELL(cellacc, ncell, 2 + oc_elem[n], k, add);
         *x = 0.0;
       }
     }
@@ -588,11 +600,11 @@ static int col_step(const oc_params *p, const colview *v, int method,
     col_rhs(p, v, c, ks, w->r[0], S, w->F[0], w->s[0][0], w->s[0][1]);
     for (int n = 0; n < NV; ++n)
       for (int k = 0; k < K; ++k) c[n * ks + k] = c[n * ks + k] + dt * w->r[0][n * S + k];
-    if (colacc) for (int n = 0; n < NV; ++n) colacc[n * ncol] += dt * w->F[0][n];
+    if (colacc) for (int n = 0; n < NV; ++n) ACC_COL(colacc, ncol, n, dt * w->F[0][n]);
     if (cellacc)
       for (int k = 0; k < K; ++k) {
-        cellacc[k] += dt * w->s[0][0][k];
-        cellacc[ncell + k] += dt * w->s[0][1][k];
+        ACC_CELL(cellacc, ncell, 0, k, dt * w->s[0][0][k]);
+        ACC_CELL(cellacc, ncell, 1, k, dt * w->s[0][1][k]);
       }
     memcpy(F1out, w->F[0], sizeof(double) * NV);
   } else if (method == OC_RK4 || method == OC_RK4_38) {
@@ -630,11 +642,11 @@ static int col_step(const oc_params *p, const colview *v, int method,
       }
     if (colacc)
       for (int n = 0; n < NV; ++n)
-        colacc[n * ncol] += wdt * COMB(w->F[0][n], w->F[1][n], w->F[2][n], w->F[3][n]);
+        ACC_COL(colacc, ncol, n, wdt * COMB(w->F[0][n], w->F[1][n], w->F[2][n], w->F[3][n]));
     if (cellacc)
       for (int a = 0; a < 2; ++a)
         for (int k = 0; k < K; ++k)
-          cellacc[a * ncell + k] += wdt * COMB(w->s[0][a][k], w->s[1][a][k], w->s[2][a][k], w->s[3][a][k]);
+          ACC_CELL(cellacc, ncell, a, k, wdt * COMB(w->s[0][a][k], w->s[1][a][k], w->s[2][a][k], w->s[3][a][k]));
 #undef COMB
     memcpy(F1out, w->F[3], sizeof(double) * NV);
   } else { /* EMP-1 / EMP-2, fixed step */
@@ -644,11 +656,11 @@ static int col_step(const oc_params *p, const colview *v, int method,
     if (method == OC_EMP1) {
       for (int n = 0; n < NV; ++n)
         for (int k = 0; k < K; ++k) c[n * ks + k] = c[n * ks + k] + (dt * p1) * w->r[0][n * S + k];
-      if (colacc) for (int n = 0; n < NV; ++n) colacc[n * ncol] += (dt * p1) * w->F[0][n];
+      if (colacc) for (int n = 0; n < NV; ++n) ACC_COL(colacc, ncol, n, (dt * p1) * w->F[0][n]);
       if (cellacc)
         for (int k = 0; k < K; ++k) {
-          cellacc[k] += (dt * p1) * w->s[0][0][k];
-          cellacc[ncell + k] += (dt * p1) * w->s[0][1][k];
+          ACC_CELL(cellacc, ncell, 0, k, (dt * p1) * w->s[0][0][k]);
+          ACC_CELL(cellacc, ncell, 1, k, (dt * p1) * w->s[0][1][k]);
         }
       memcpy(F1out, w->F[0], sizeof(double) * NV);
     } else {
@@ -666,11 +678,11 @@ static int col_step(const oc_params *p, const colview *v, int method,
       for (int n = 0; n < NV; ++n)
         for (int k = 0; k < K; ++k) c[n * ks + k] = c[n * ks + k] + (dt * p2) * w->r[2][n * S + k];
       if (colacc)
-        for (int n = 0; n < NV; ++n) colacc[n * ncol] += (dt * p2) * ((w->F[0][n] + w->F[1][n]) * 0.5);
+        for (int n = 0; n < NV; ++n) ACC_COL(colacc, ncol, n, (dt * p2) * ((w->F[0][n] + w->F[1][n]) * 0.5));
       if (cellacc)
         for (int a = 0; a < 2; ++a)
           for (int k = 0; k < K; ++k)
-            cellacc[a * ncell + k] += (dt * p2) * ((w->s[0][a][k] + w->s[1][a][k]) * 0.5);
+            ACC_CELL(cellacc, ncell, a, k, (dt * p2) * ((w->s[0][a][k] + w->s[1][a][k]) * 0.5));
       memcpy(F1out, w->F[1], sizeof(double) * NV);
     }
     /* Patankar keeps components non-negative up to rounding */
@@ -759,18 +771,18 @@ static int col_emp_adaptive(const oc_params *p, const colview *v, double *c,
         for (int k = 0; k < K; ++k) {
           double y = cn[n * S + k];
           if (y < 0.0) {
-            if (cellacc && oc_elem[n] >= 0) cellacc[(2 + oc_elem[n]) * ncell + k] += 0.0 - y;
+            if (cellacc && oc_elem[n] >= 0) ACC_CELL(cellacc, ncell, 2 + oc_elem[n], k, 0.0 - y);
             y = 0.0;
           }
           c[n * ks + k] = y;
         }
       }
       if (colacc)
-        for (int n = 0; n < NV; ++n) colacc[n * ncol] += (h * p2) * ((w->F[0][n] + w->F[1][n]) * 0.5);
+        for (int n = 0; n < NV; ++n) ACC_COL(colacc, ncol, n, (h * p2) * ((w->F[0][n] + w->F[1][n]) * 0.5));
       if (cellacc)
         for (int a = 0; a < 2; ++a)
           for (int k = 0; k < K; ++k)
-            cellacc[a * ncell + k] += (h * p2) * ((w->s[0][a][k] + w->s[1][a][k]) * 0.5);
+            ACC_CELL(cellacc, ncell, a, k, (h * p2) * ((w->s[0][a][k] + w->s[1][a][k]) * 0.5));
       memcpy(F1out, w->F[1], sizeof(double) * NV);
       rem = rem - h;
       dg->accepted += 1;
@@ -883,11 +895,11 @@ int oc_ode_solver(const oc_params *p, const oc_domain *d, double *conc,
           int o = d->off[c], K = d->off[c + 1] - o;
           for (int n = 0; n < NV; ++n)
             for (int k = 0; k < K; ++k) conc[n * ncell + o + k] = c1[n * ncell + o + k];
-          if (colacc) for (int n = 0; n < NV; ++n) colacc[n * ncol + c] += dt_red * F[n * ncol + c];
+          if (colacc) for (int n = 0; n < NV; ++n) ACC_COL(colacc + c, ncol, n, dt_red * F[n * ncol + c]);
           if (cellacc)
             for (int k = 0; k < K; ++k) {
-              cellacc[o + k] += dt_red * s[o + k];
-              cellacc[ncell + o + k] += dt_red * s[ncell + o + k];
+              ACC_CELL(cellacc + o, ncell, 0, k, dt_red * s[o + k]);
+              ACC_CELL(cellacc + o, ncell, 1, k, dt_red * s[ncell + o + k]);
             }
           if (diag) {
             diag[c].accepted += 1;
@@ -1034,10 +1046,12 @@ int oc_budget(const oc_domain *d, const double *cellacc, const double *colacc,
   for (int c = 0; c < ncol; ++c) {
     int o = d->off[c], K = d->off[c + 1] - o;
     double *b = budget + (long)c * OC_NBUD;
-    for (int n = 0; n < NV; ++n) b[n] = colacc ? colacc[(long)n * ncol + c] : 0.0;
+    for (int n = 0; n < NV; ++n)
+      b[n] = colacc ? colacc[(long)n * ncol + c] + colacc[(long)(NV + n) * ncol + c] : 0.0;
     for (int a = 0; a < 5; ++a) {
       double t[OC_KMAX];
-      for (int k = 0; k < K; ++k) t[k] = (d->phi[o + k] * d->dz[o + k]) * cellacc[a * ncell + o + k];
+      for (int k = 0; k < K; ++k)
+        t[k] = (d->phi[o + k] * d->dz[o + k]) * (cellacc[a * ncell + o + k] + cellacc[(5 + a) * ncell + o + k]);
       b[NV + a] = tree_sum(t, K);
     }
   }

@@ -376,3 +376,56 @@ def test_profile3_bioturbation_bitwise(gpu_ready, arith):
     # switched off: the set_geometry factor again
     assert np.array_equal(r_off, O.Oracle(dom, p).rhs(x)[0])
     assert not np.array_equal(r_off, ro)
+
+
+@pytest.mark.parametrize("arith", ARITHS)
+def test_rk4_one_year(gpu_ready, arith):
+    """The plain 4x4x30 case run for one simulated year (87 600 RK4 steps of
+    360 s, minimum clamp after each): relative 1e-8 against the oracle
+    (bitwise in fact) and C/N/P budget closure to 1e-12."""
+    dom = cases.plain_columns(seed=0)
+    dom.params.arith = arith
+    p = dom.params.si()
+    nsteps = 365 * 240
+    with capi.SedimentColumns.from_domain(dom) as g:
+        g.run(360.0, nsteps, RUNGE_KUTTA_4)
+        xg = g.get_state()
+        bg = g.budget()
+        invg = g.inventory()
+    orc = O.Oracle(dom, p)
+    xo = dom.conc0.copy()
+    assert orc.run(xo, 360.0, nsteps, RUNGE_KUTTA_4, nthreads=8) == 0
+    err = max_rel(xg, xo)
+    print(f"RK4 1 year max rel {err:.3e}", flush=True)
+    assert err <= 1e-8
+    assert np.array_equal(xg, xo)
+    res = budget_residuals(orc.inventory(dom.conc0), invg, bg)
+    for e, r in res.items():
+        print(f"budget {e}: max rel closure {r.max():.3e}", flush=True)
+        assert r.max() <= 1e-12
+
+
+def test_shelf_sea_full_size_properties(gpu_ready):
+    """The full 300 x 300 shelf-sea domain (53 211 wet columns), first
+    simulated day from the cold state under adaptive EMP: every
+    concentration finite and non-negative, land untouched, and the C/N/P
+    budget of every column closed to 1e-12 relative (inventory change =
+    integrated top fluxes - DIC/N2 release + clamp mass)."""
+    dom = cases.shelf_sea(n=300, days=2, seed=0)
+    dom.params.arith = 1
+    land = np.repeat(dom.mask, dom.nlayers) != 0
+    with capi.SedimentColumns.from_domain(dom) as g:
+        inv0 = g.inventory()
+        g.integrate(0.0, 86400.0)
+        x = g.get_state()
+        b = g.budget()
+        inv1 = g.inventory()
+        d = g.diagnostics()
+    wet = dom.mask == 0
+    assert np.all(np.isfinite(x[:, ~land])) and np.all(x[:, ~land] >= 0.0)
+    assert np.array_equal(x[:, land], dom.conc0[:, land])
+    assert np.all(d["status"][wet] == 0) and np.all(d["accepted"][wet] > 0)
+    res = budget_residuals(inv0, inv1, b)
+    for e, r in res.items():
+        print(f"budget {e}: max rel closure {r[wet].max():.3e}", flush=True)
+        assert r[wet].max() <= 1e-12
