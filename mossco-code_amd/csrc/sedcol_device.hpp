@@ -231,11 +231,13 @@ __device__ __forceinline__ void kinetics(const sedcol_params &p, double fT, cons
 // lane exchange
 // ---------------------------------------------------------------------------
 
+// bound_ctrl set: a lane whose source is out of range or disabled reads 0,
+// so no destination register needs clearing first
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v)
 {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 
@@ -479,8 +481,19 @@ __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g
   const double Sc = STAGE2 ? lg_prod<LPC>(Sl) : 1.0;
   // quadratic convergence from below; stop once a correction is below
   // 2^-20 p (remaining error ~1e-12 p, positivity and conservation intact)
-  double pp = 0.0;
-  for (int it = 0; it < 60; ++it) {
+  // First step, from p = 0: every factor is 1, so the product is 1 and its
+  // derivative the plain sum of the a_j.  The butterfly sum below takes the
+  // same operands in the same order as lg_pd would (x * 1 is exact), so the
+  // iterate equals the general step's bit for bit.
+  double D0 = 0.0;
+#pragma unroll
+  for (int n = 0; n < NV; ++n) D0 = D0 + a[n];
+  D0 = lg_sum<LPC>(D0);
+  newton += 1;
+  double pp = 0.0 - Sc / (Sc * D0 - 1.0);
+  if (!(pp > 0.0)) return 0.0;
+  if (pp <= 0x1p-20 * pp) return (!STAGE2 && pp > 1.0) ? 1.0 : pp;
+  for (int it = 1; it < 60; ++it) {
     newton += 1;
     double P = 1.0, D = 0.0;
 #pragma unroll
