@@ -196,8 +196,9 @@ def main(argv=None):
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(columns are independent: see DESIGN.md for how the domain would shard).", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(columns are independent: a multi-GPU run would give each process its own share of the "
+              "wet columns).", file=sys.stderr)
         return 2
     if args.workload == "ragged":
         return ragged_bench(args)

@@ -27,6 +27,9 @@
  *     (fabm_sediment_component.F90:538-541).
  *   - a handle owns its device buffers and one HIP stream; it is not
  *     thread-safe (one handle per host thread, as type_sed is per PET).
+ *   - a handle belongs to the device that is current when sedcol_create is
+ *     called (hipSetDevice beforehand selects it); every call runs on that
+ *     device and leaves the caller's current device as it found it.
  */
 #ifndef SEDCOL_H
 #define SEDCOL_H

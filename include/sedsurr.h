@@ -12,7 +12,10 @@
  * matrix products on the MFMA units with FP32 accumulation, biases, GELU
  * (tanh form), loss, gradients and Adam state in FP32.  Inputs and targets
  * cross the ABI as FP32, already normalised by the caller; every function
- * returns a sedcol_status code (sedcol.h) and sets sedsurr_last_error(). */
+ * returns a sedcol_status code (sedcol.h) and sets sedsurr_last_error().
+ * A handle belongs to the device that is current at sedsurr_create; every
+ * call runs on that device and leaves the caller's current device as it
+ * found it. */
 #ifndef SEDSURR_H
 #define SEDSURR_H
 

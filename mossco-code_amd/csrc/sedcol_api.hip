@@ -38,6 +38,7 @@ constexpr int kLpc[3] = {16, 32, 64};
 }  // namespace
 
 struct sedcol_handle {
+  int device = 0;  // current at sedcol_create; every call runs on it
   int ncol = 0, kmax = 0, layout = 0;
   long long ncell = 0;
   std::vector<int> off;   // ncol+1
@@ -178,6 +179,24 @@ void unpack_state(const sedcol_handle *h, const std::vector<double> &src, double
       for (int c = 0; c < h->ncol; ++c)
         dst[n * plane + fidx(h, c, k)] = src[n * h->ncell + (long long)c * h->kmax + k];
 }
+
+// Makes the handle's device current for the duration of one entry point and
+// restores the caller's current device afterwards: a handle belongs to the
+// device that was current at sedcol_create.
+struct DeviceScope {
+  int prev = -1, dev;
+  explicit DeviceScope(int d) : dev(d)
+  {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceScope()
+  {
+    if (prev >= 0 && prev != dev) (void)hipSetDevice(prev);
+  }
+  DeviceScope(const DeviceScope &) = delete;
+  DeviceScope &operator=(const DeviceScope &) = delete;
+};
 
 int check_handle(sedcol_handle *h, bool need_state_inputs)
 {
@@ -498,7 +517,7 @@ int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
     if (e_ != hipSuccess)                                                               \
       return cleanup(fail(SEDCOL_EHIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
   } while (0)
-  HIPOK_C(hipSetDevice(0));
+  HIPOK_C(hipGetDevice(&h->device));
   HIPOK_C(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPOK_C(hipEventCreate(&h->ev0));
   HIPOK_C(hipEventCreate(&h->ev1));
@@ -552,6 +571,7 @@ int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
 int sedcol_destroy(sedcol_handle *h)
 {
   if (!h) return SEDCOL_OK;
+  const DeviceScope dscope(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void *ptrs[] = {h->d_off, h->d_list[0], h->d_list[1], h->d_list[2], h->d_dz, h->d_dzc, h->d_phi, h->d_phiI,
                   h->d_bfac, h->d_biot, h->d_temp, h->d_cbw, h->d_fup, h->d_conc, h->d_cellacc, h->d_colacc,
@@ -577,6 +597,7 @@ int sedcol_destroy(sedcol_handle *h)
 int sedcol_set_params(sedcol_handle *h, const sedcol_params *p)
 {
   if (!h || !p) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   if (!(p->bcup_dis >= 1 && p->bcup_dis <= 3)) return fail(SEDCOL_EINVAL, "bcup_dis must be 1, 2 or 3");
   if (!(p->bcup_part >= 1 && p->bcup_part <= 4)) return fail(SEDCOL_EINVAL, "bcup_part must be 1, 2, 3 or 4");
   if (!(p->bcdown >= 1 && p->bcdown <= 3)) return fail(SEDCOL_EINVAL, "bcdown must be 1, 2 or 3");
@@ -611,6 +632,7 @@ int zw_geometry(sedcol_handle *h)
 int sedcol_set_bioturbation_zw(sedcol_handle *h, const sedcol_bioturbation_zw *zw)
 {
   if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  const DeviceScope dscope(h->device);
   if (!zw) {
     h->zw_on = false;
     h->p.arith = h->user_arith;
@@ -644,6 +666,7 @@ int sedcol_set_geometry(sedcol_handle *h, const double *dz, const double *dzc, c
 {
   if (!h || !dz || !dzc || !phi || !phi_intf || !bioturb_factor || !bioturbation)
     return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   std::vector<double> a[5];
   pack_cells(h, dz, a[0]);
   pack_cells(h, dzc, a[1]);
@@ -687,6 +710,7 @@ int sedcol_set_forcing_series(sedcol_handle *h, int32_t nrec, double t0, double 
                               const double *c_bw, const double *flux_top, int32_t nrec_flux)
 {
   if (!h || !temp || !c_bw || !flux_top) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   if (nrec < 1) return fail(SEDCOL_EINVAL, "nrec must be positive");
   if (nrec_flux != 1 && nrec_flux != nrec) return fail(SEDCOL_EINVAL, "nrec_flux must be 1 or nrec");
   if (nrec > 1 && !(dt_rec > 0.0 && std::isfinite(dt_rec))) return fail(SEDCOL_EINVAL, "dt_rec must be positive");
@@ -714,6 +738,7 @@ int sedcol_set_forcing(sedcol_handle *h, const double *temp, const double *c_bw,
 int sedcol_set_flux_cap(sedcol_handle *h, const double *flux_cap)
 {
   if (!h || !flux_cap) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   std::vector<double> cap;
   pack_cells(h, flux_cap, cap);
   for (int c = 0; c < h->ncol; ++c) {
@@ -731,6 +756,7 @@ int sedcol_assemble_forcing(sedcol_handle *h, const double *temp_surface, const 
                             const double *w_surface, const int32_t *present, double *flux_out)
 {
   if (!h || !present) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   if (!h->have_params) return fail(SEDCOL_EINVAL, "sedcol_set_params has not been called");
   if (!h->have_geom) return fail(SEDCOL_EINVAL, "sedcol_set_geometry has not been called");
   if (h->nrec > 1) return fail(SEDCOL_EINVAL, "a forcing series is active; boundary assembly sets one record");
@@ -797,6 +823,7 @@ int sedcol_assemble_forcing(sedcol_handle *h, const double *temp_surface, const 
 int sedcol_set_state(sedcol_handle *h, const double *conc)
 {
   if (!h || !conc) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   std::vector<double> buf;
   pack_state(h, conc, buf);
   HIPOK(hipMemcpyAsync(h->d_conc, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice, h->stream));
@@ -807,6 +834,7 @@ int sedcol_set_state(sedcol_handle *h, const double *conc)
 int sedcol_get_state(sedcol_handle *h, double *conc)
 {
   if (!h || !conc) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   std::vector<double> buf((size_t)NV * h->ncell);
   HIPOK(hipMemcpyAsync(buf.data(), h->d_conc, sizeof(double) * buf.size(), hipMemcpyDeviceToHost, h->stream));
   HIPOK(hipStreamSynchronize(h->stream));
@@ -819,6 +847,7 @@ int sedcol_rhs(sedcol_handle *h, const double *conc, double *rhs, double *flux_t
 {
   int rc = check_handle(h, true);
   if (rc) return rc;
+  const DeviceScope dscope(h->device);
   if (!conc || !rhs) return fail(SEDCOL_EINVAL, "null argument");
   if ((rc = ensure_scratch(h))) return rc;
   std::vector<double> buf;
@@ -911,6 +940,7 @@ int sedcol_run(sedcol_handle *h, double dt, int32_t nsteps, int32_t method, int3
 {
   int rc = check_handle(h, true);
   if (rc) return rc;
+  const DeviceScope dscope(h->device);
   if (nsteps < 0) return fail(SEDCOL_EINVAL, "nsteps must be >= 0");
   if (!(dt > 0.0) || !std::isfinite(dt)) return fail(SEDCOL_EINVAL, "dt must be positive");
   if (method < SEDCOL_EULER || method > SEDCOL_EMP2)
@@ -949,6 +979,7 @@ int sedcol_integrate(sedcol_handle *h, double t_start, double t_end)
 {
   int rc = check_handle(h, true);
   if (rc) return rc;
+  const DeviceScope dscope(h->device);
   if (!std::isfinite(t_start) || !std::isfinite(t_end) || !(t_end >= t_start))
     return fail(SEDCOL_EINVAL, "need finite t_start <= t_end");
   int launches = 0;
@@ -983,6 +1014,7 @@ int sedcol_integrate(sedcol_handle *h, double t_start, double t_end)
 int sedcol_set_schedule(sedcol_handle *h, const sedcol_schedule *s)
 {
   if (!h || !s) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   if (s->window < 0) return fail(SEDCOL_EINVAL, "window must be >= 0");
   if (s->graph_windows < 0 || s->graph_windows > 4096) return fail(SEDCOL_EINVAL, "graph_windows outside 0..4096");
   if (s->check_every < 1) return fail(SEDCOL_EINVAL, "check_every must be >= 1");
@@ -995,6 +1027,7 @@ int sedcol_set_schedule(sedcol_handle *h, const sedcol_schedule *s)
 int sedcol_get_schedule(sedcol_handle *h, sedcol_schedule *s)
 {
   if (!h || !s) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   *s = h->sched;
   return SEDCOL_OK;
 }
@@ -1002,6 +1035,7 @@ int sedcol_get_schedule(sedcol_handle *h, sedcol_schedule *s)
 int sedcol_set_tuning(sedcol_handle *h, const sedcol_tuning *t)
 {
   if (!h || !t) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   int rc = check_tuning(h, t);
   if (rc) return rc;
   for (int b = 0; b < SEDCOL_NBIN; ++b) {
@@ -1015,6 +1049,7 @@ int sedcol_set_tuning(sedcol_handle *h, const sedcol_tuning *t)
 int sedcol_get_tuning(sedcol_handle *h, sedcol_tuning *t)
 {
   if (!h || !t) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   for (int b = 0; b < SEDCOL_NBIN; ++b) {
     t->lanes_per_column[b] = h->lpc[b];
     t->columns_per_group[b] = h->blk[b] / h->lpc[b];
@@ -1027,6 +1062,7 @@ int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t rep
 {
   int rc = check_handle(h, true);
   if (rc) return rc;
+  const DeviceScope dscope(h->device);
   if (!best) return fail(SEDCOL_EINVAL, "null argument");
   if (!std::isfinite(t_start) || !(t_span > 0.0) || !std::isfinite(t_span))
     return fail(SEDCOL_EINVAL, "need finite t_start and t_span > 0");
@@ -1124,6 +1160,7 @@ int sedcol_step(sedcol_handle *h, double dt, int32_t method, double *conc_inout,
 {
   int rc = check_handle(h, true);
   if (rc) return rc;
+  const DeviceScope dscope(h->device);
   if (!conc_inout) return fail(SEDCOL_EINVAL, "null argument");
   if (method < SEDCOL_EULER || method > SEDCOL_EMP_ADAPTIVE) return fail(SEDCOL_EINVAL, "unknown method");
   if ((rc = sedcol_set_state(h, conc_inout))) return rc;
@@ -1146,6 +1183,7 @@ int sedcol_spinup(sedcol_handle *h, double t_start, double chunk, double t_max, 
 {
   int rc = check_handle(h, true);
   if (rc) return rc;
+  const DeviceScope dscope(h->device);
   if (!cfloor) return fail(SEDCOL_EINVAL, "null cfloor");
   if (!std::isfinite(t_start) || !(chunk > 0.0) || !std::isfinite(chunk) || !(t_max >= 0.0) ||
       !std::isfinite(t_max) || !(tol > 0.0))
@@ -1212,6 +1250,7 @@ int sedcol_spinup(sedcol_handle *h, double t_start, double chunk, double t_max, 
 int sedcol_checkpoint_size(sedcol_handle *h, int64_t *bytes)
 {
   if (!h || !bytes) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   *bytes = ck_bytes(h);
   return SEDCOL_OK;
 }
@@ -1219,6 +1258,7 @@ int sedcol_checkpoint_size(sedcol_handle *h, int64_t *bytes)
 int sedcol_save_checkpoint(sedcol_handle *h, void *buf, int64_t bytes)
 {
   if (!h || !buf) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   if (bytes < ck_bytes(h)) return fail(SEDCOL_EINVAL, "checkpoint buffer too small (sedcol_checkpoint_size)");
   CkHeader hd;
   std::memcpy(hd.magic, kCkMagic, 8);
@@ -1239,6 +1279,7 @@ int sedcol_save_checkpoint(sedcol_handle *h, void *buf, int64_t bytes)
 int sedcol_load_checkpoint(sedcol_handle *h, const void *buf, int64_t bytes)
 {
   if (!h || !buf) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   if (bytes < (int64_t)sizeof(CkHeader)) return fail(SEDCOL_EINVAL, "checkpoint too short");
   CkHeader hd;
   std::memcpy(&hd, buf, sizeof hd);
@@ -1259,6 +1300,7 @@ int sedcol_load_checkpoint(sedcol_handle *h, const void *buf, int64_t bytes)
 int sedcol_get_fluxes(sedcol_handle *h, double *flux_top_out)
 {
   if (!h || !flux_top_out) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   HIPOK(hipMemcpyAsync(flux_top_out, h->d_flux, sizeof(double) * NV * h->ncol, hipMemcpyDeviceToHost, h->stream));
   HIPOK(hipStreamSynchronize(h->stream));
   return SEDCOL_OK;
@@ -1267,6 +1309,7 @@ int sedcol_get_fluxes(sedcol_handle *h, double *flux_top_out)
 int sedcol_get_budget(sedcol_handle *h, double *budget)
 {
   if (!h || !budget) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   double *d_b = nullptr;
   HIPOK(hipMalloc(&d_b, sizeof(double) * SEDCOL_NBUD * h->ncol));
   hipError_t e = hipMemsetAsync(d_b, 0, sizeof(double) * SEDCOL_NBUD * h->ncol, h->stream);
@@ -1285,6 +1328,7 @@ int sedcol_get_budget(sedcol_handle *h, double *budget)
 int sedcol_reset_budget(sedcol_handle *h)
 {
   if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  const DeviceScope dscope(h->device);
   HIPOK(hipMemsetAsync(h->d_cellacc, 0, sizeof(double) * 10 * (size_t)h->ncell, h->stream));
   HIPOK(hipMemsetAsync(h->d_colacc, 0, sizeof(double) * 2 * NV * h->ncol, h->stream));
   HIPOK(hipStreamSynchronize(h->stream));
@@ -1294,6 +1338,7 @@ int sedcol_reset_budget(sedcol_handle *h)
 int sedcol_get_inventory(sedcol_handle *h, double *inventory)
 {
   if (!h || !inventory) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   HIPOK(hipMemsetAsync(h->d_small, 0, sizeof(double) * NV * h->ncol, h->stream));
   const Dev d = h->dev();
   for (int b = 0; b < 3; ++b)
@@ -1308,6 +1353,7 @@ int sedcol_get_inventory(sedcol_handle *h, double *inventory)
 int sedcol_get_diagnostics(sedcol_handle *h, sedcol_coldiag *diag)
 {
   if (!h || !diag) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   HIPOK(hipMemcpyAsync(diag, h->d_diag, sizeof(sedcol_coldiag) * h->ncol, hipMemcpyDeviceToHost, h->stream));
   HIPOK(hipStreamSynchronize(h->stream));
   return SEDCOL_OK;
@@ -1316,6 +1362,7 @@ int sedcol_get_diagnostics(sedcol_handle *h, sedcol_coldiag *diag)
 int sedcol_reset_diagnostics(sedcol_handle *h)
 {
   if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  const DeviceScope dscope(h->device);
   h->last_min_dt = 1.e20;
   return reset_diag(h);
 }
@@ -1324,6 +1371,7 @@ int sedcol_residual(sedcol_handle *h, const double *cfloor, double *resid)
 {
   int rc = check_handle(h, true);
   if (rc) return rc;
+  const DeviceScope dscope(h->device);
   if (!cfloor || !resid) return fail(SEDCOL_EINVAL, "null argument");
   return residual_flux(h, cfloor, resid, nullptr);
 }
@@ -1331,6 +1379,7 @@ int sedcol_residual(sedcol_handle *h, const double *cfloor, double *resid)
 int sedcol_get_last_min_dt(sedcol_handle *h, double *last_min_dt)
 {
   if (!h || !last_min_dt) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   *last_min_dt = h->last_min_dt;
   return SEDCOL_OK;
 }
@@ -1338,6 +1387,7 @@ int sedcol_get_last_min_dt(sedcol_handle *h, double *last_min_dt)
 int sedcol_synchronize(sedcol_handle *h)
 {
   if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  const DeviceScope dscope(h->device);
   HIPOK(hipStreamSynchronize(h->stream));
   return SEDCOL_OK;
 }
@@ -1345,6 +1395,7 @@ int sedcol_synchronize(sedcol_handle *h)
 int sedcol_last_timing(sedcol_handle *h, double *ms, int32_t *launches)
 {
   if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  const DeviceScope dscope(h->device);
   if (ms) *ms = h->last_ms;
   if (launches) *launches = h->last_launches;
   return SEDCOL_OK;

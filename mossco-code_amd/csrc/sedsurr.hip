@@ -306,7 +306,26 @@ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 }  // namespace
 
+// Makes the handle's device current for the duration of one entry point and
+// restores the caller's current device afterwards: a handle belongs to the
+// device that was current at sedsurr_create.
+struct DeviceScope {
+  int prev = -1, dev;
+  explicit DeviceScope(int d) : dev(d)
+  {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceScope()
+  {
+    if (prev >= 0 && prev != dev) (void)hipSetDevice(prev);
+  }
+  DeviceScope(const DeviceScope &) = delete;
+  DeviceScope &operator=(const DeviceScope &) = delete;
+};
+
 struct sedsurr_handle {
+  int device = 0;  // current at sedsurr_create; every call runs on it
   sedsurr_shape s{};
   int kin = 0, nout_p = 0, nl = 0;  // padded input width, padded output width, weight layers
   std::vector<int> fi, fo, fip, fop;    // logical and padded fan-in / fan-out per layer
@@ -516,7 +535,7 @@ int sedsurr_create(const sedsurr_shape *shape, sedsurr_handle **out)
     if (e_ != hipSuccess)                                                                   \
       return cleanup(fail(SEDCOL_EHIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
   } while (0)
-  HIPOK_C(hipSetDevice(0));
+  HIPOK_C(hipGetDevice(&h->device));
   HIPOK_C(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   HIPOK_C(hipEventCreate(&h->e0));
   HIPOK_C(hipEventCreate(&h->e1));
@@ -556,6 +575,7 @@ int sedsurr_create(const sedsurr_shape *shape, sedsurr_handle **out)
 int sedsurr_destroy(sedsurr_handle *h)
 {
   if (!h) return SEDCOL_OK;
+  const DeviceScope dscope(h->device);
   if (h->st) (void)hipStreamSynchronize(h->st);
   std::vector<void *> ptrs = {h->w, h->g, h->m, h->v, h->wb, h->xb, h->yh, h->yb, h->dz0, h->dz1,
                               h->part, h->loss, h->X, h->Y, h->rows, h->xin, h->yin, h->kpart, h->cpart};
@@ -573,6 +593,7 @@ int sedsurr_destroy(sedsurr_handle *h)
 int sedsurr_num_params(sedsurr_handle *h, int64_t *n)
 {
   if (!h || !n) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   *n = h->np;
   return SEDCOL_OK;
 }
@@ -580,6 +601,7 @@ int sedsurr_num_params(sedsurr_handle *h, int64_t *n)
 int sedsurr_set_params(sedsurr_handle *h, const float *params)
 {
   if (!h || !params) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   for (long long i = 0; i < h->np; ++i)
     if (!std::isfinite(params[i])) return fail(SEDCOL_EDOMAIN, "non-finite parameter");
   std::vector<float> buf;
@@ -598,6 +620,7 @@ int sedsurr_set_params(sedsurr_handle *h, const float *params)
 int sedsurr_get_params(sedsurr_handle *h, float *params)
 {
   if (!h || !params) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   std::vector<float> buf(h->np_pad);
   HIPOK(hipMemcpyAsync(buf.data(), h->w, sizeof(float) * h->np_pad, hipMemcpyDeviceToHost, h->st));
   HIPOK(hipStreamSynchronize(h->st));
@@ -608,6 +631,7 @@ int sedsurr_get_params(sedsurr_handle *h, float *params)
 int sedsurr_set_data(sedsurr_handle *h, const float *X, const float *Y, int64_t n)
 {
   if (!h || !X || !Y) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   if (n < 1) return fail(SEDCOL_EINVAL, "need n >= 1");
   if (h->X) (void)hipFree(h->X);
   if (h->Y) (void)hipFree(h->Y);
@@ -626,6 +650,7 @@ int sedsurr_train(sedsurr_handle *h, int32_t steps, int32_t batch, const int32_t
                   float *loss)
 {
   if (!h || !rows || !adam) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   if (!h->X) return fail(SEDCOL_EINVAL, "no training set (sedsurr_set_data)");
   if (steps < 1) return fail(SEDCOL_EINVAL, "need steps >= 1");
   if (batch < 1 || batch > h->s.max_batch) return fail(SEDCOL_EINVAL, "batch outside 1..max_batch");
@@ -682,6 +707,7 @@ int sedsurr_train(sedsurr_handle *h, int32_t steps, int32_t batch, const int32_t
 int sedsurr_loss_grad(sedsurr_handle *h, const float *X, const float *Y, int32_t n, float *loss, float *grad)
 {
   if (!h || !X || !Y || !loss || !grad) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   if (n < 1 || n > h->s.max_batch) return fail(SEDCOL_EINVAL, "n outside 1..max_batch");
   HIPOK(hipMemcpyAsync(h->xin, X, sizeof(float) * (size_t)n * h->s.n_in, hipMemcpyHostToDevice, h->st));
   HIPOK(hipMemcpyAsync(h->yin, Y, sizeof(float) * (size_t)n * h->s.n_out, hipMemcpyHostToDevice, h->st));
@@ -702,6 +728,7 @@ int sedsurr_loss_grad(sedsurr_handle *h, const float *X, const float *Y, int32_t
 int sedsurr_predict(sedsurr_handle *h, const float *X, int64_t n, float *Y)
 {
   if (!h || !X || !Y) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   if (n < 0) return fail(SEDCOL_EINVAL, "need n >= 0");
   int launches = 0;
   HIPOK(hipEventRecord(h->e0, h->st));
@@ -729,6 +756,7 @@ int sedsurr_predict(sedsurr_handle *h, const float *X, int64_t n, float *Y)
 int sedsurr_last_timing(sedsurr_handle *h, double *ms, int32_t *launches)
 {
   if (!h || !ms || !launches) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
   *ms = h->last_ms;
   *launches = h->last_launches;
   return SEDCOL_OK;

@@ -179,13 +179,18 @@ class SedimentColumns:
     def __init__(self, nlayers, mask=None, layout="packed"):
         self.nlayers = _c(nlayers, np.int32)
         self.ncol = int(self.nlayers.size)
-        self.kmax = int(self.nlayers.max())
-        self.layout = LAYOUT_PACKED if layout == "packed" else LAYOUT_FORTRAN
+        self.kmax = int(self.nlayers.max()) if self.nlayers.size else 0
+        # an unknown layout name goes to the library as -1, which rejects it
+        self.layout = {"packed": LAYOUT_PACKED, "fortran": LAYOUT_FORTRAN}.get(layout, -1)
         self.mask = _c(np.zeros(self.ncol) if mask is None else mask, np.int32)
         self.ncell = int(self.nlayers.sum())
         d = Desc()
         d.ncol, d.kmax, d.nvar, d.layout = self.ncol, self.kmax, NVAR, self.layout
-        d.nlayers = self.nlayers.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if self.layout == LAYOUT_PACKED else None
+        # the Fortran layout takes its uniform K from kmax; ragged counts are
+        # passed on so that the library rejects them
+        uniform = self.nlayers.size > 0 and np.all(self.nlayers == self.kmax)
+        d.nlayers = (None if self.layout == LAYOUT_FORTRAN and uniform
+                     else self.nlayers.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
         d.mask = self.mask.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         h = ctypes.c_void_p()
         check(lib().sedcol_create(ctypes.byref(d), ctypes.byref(h)))

@@ -46,3 +46,24 @@ def test_create_without_device_fails_cleanly():
         assert e.code == capi.SEDCOL_EHIP
     else:
         raise AssertionError("handle creation succeeded without a device")
+
+
+def test_descriptor_errors_need_no_device():
+    """Descriptor checks run before any HIP call: an empty domain, layer
+    counts outside 1..64 and a ragged Fortran layout are argument errors."""
+    import numpy as np
+
+    for nl in (np.zeros(0, np.int32), np.array([0, 5]), np.array([65]), np.array([3, 70])):
+        try:
+            capi.SedimentColumns(nl)
+        except capi.SedcolError as e:
+            assert e.code == capi.SEDCOL_EINVAL, (nl, e)
+        else:
+            raise AssertionError(f"nlayers {nl.tolist()} accepted")
+    for nl, layout in ((np.array([3, 4]), "fortran"), (np.array([3, 3]), "columns")):
+        try:
+            capi.SedimentColumns(nl, layout=layout)
+        except capi.SedcolError as e:
+            assert e.code == capi.SEDCOL_EINVAL, (layout, e)
+        else:
+            raise AssertionError(f"layout {layout} with nlayers {nl.tolist()} accepted")

@@ -1,6 +1,6 @@
 """GPU parity: libsedcol.so (through the C ABI) against the CPU oracle.
 
-Tolerances (north star): 1e-10 relative per RK4 step, 1e-8 after a
+Tolerances: 1e-10 relative per RK4 step, 1e-8 after a
 simulated year, C/N/P budget closure 1e-12 relative, non-negative state
 under EMP.  The kernels and the oracle execute the same IEEE operation
 sequence, so most comparisons here are bitwise; the stated tolerance is
@@ -189,6 +189,47 @@ def test_emp_ragged(gpu_ready, arith):
     xo = dom.conc0.copy()
     assert orc.emp_integrate(xo, span, nthreads=8) == 0
     assert np.array_equal(xg, xo)
+    assert np.all(xg >= 0.0)
+
+
+def _edge_columns():
+    """One column at each layer count where the lane mapping changes: a
+    single layer, two, the 16/32/64-lane bin edges and the maximum of 64."""
+    K = np.array([1, 2, 15, 16, 17, 31, 32, 33, 63, 64], dtype=np.int32)
+    rng = np.random.Generator(np.random.Philox(11))
+    s = cases.draw_scalars(rng, K.size)
+    from sedcol.params import SedParams
+    p = SedParams(bcup_dis=2, bcup_part=1, bcdown=3, clamp=0, emp_rtol=1e-6, emp_atol=1e-6,
+                  emp_dt_min=1.0, emp_dt_max=3600.0)
+    return cases.build_columns(K, np.linspace(0.05, 0.5, K.size), s, params=p)
+
+
+@pytest.mark.parametrize("arith", ARITHS)
+def test_layer_count_edges(gpu_ready, arith):
+    dom = _edge_columns()
+    dom.params.arith = arith
+    p = dom.params.si()
+    # RK4, one simulated day in one launch
+    with capi.SedimentColumns.from_domain(dom) as g:
+        g.run(360.0, 240, RUNGE_KUTTA_4)
+        xg = g.get_state()
+        bg = g.budget()
+    orc = O.Oracle(dom, p)
+    xo = dom.conc0.copy()
+    assert orc.run(xo, 360.0, 240, RUNGE_KUTTA_4, nthreads=8) == 0
+    assert np.array_equal(xg, xo)
+    assert np.allclose(bg, orc.budget(), rtol=1e-12, atol=1e-300)
+    # adaptive EMP, six hours
+    with capi.SedimentColumns.from_domain(dom) as g:
+        g.integrate(0.0, 6 * 3600.0)
+        xg = g.get_state()
+        dg = g.diagnostics()
+    orc = O.Oracle(dom, p)
+    xo = dom.conc0.copy()
+    assert orc.emp_integrate(xo, 6 * 3600.0, nthreads=8) == 0
+    print("edge columns: EMP attempts", (dg["accepted"] + dg["rejected"]).tolist(), flush=True)
+    assert np.array_equal(xg, xo)
+    assert np.array_equal(dg["accepted"], orc.diag["accepted"])
     assert np.all(xg >= 0.0)
 
 
