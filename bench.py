@@ -96,6 +96,7 @@ def ragged_bench(args):
     t = time.perf_counter()
     dom = cases.ragged_columns(ncol=args.ncol, seed=args.seed)
     dom.params.arith = args.arith
+    dom.params.emp_weights = args.emp_weights
     gen_s = time.perf_counter() - t
     K = dom.nlayers
     span = args.span_hours * 3600.0
@@ -132,7 +133,7 @@ def ragged_bench(args):
             rows.append({
                 "schedule": name, "shape": label, "wall_s": wall, "kernel_ms": ms, "launches": nl,
                 "column_steps_per_s": float(acc.sum()) / wall,
-                "tflops": flops.emp_flops(K, att, newt) / wall / 1e12,
+                "tflops": flops.emp_flops(K, att, newt, weights=args.emp_weights) / wall / 1e12,
                 "state_sha1": hashlib.sha1(x.tobytes()).hexdigest()[:16],
             })
             print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
@@ -162,6 +163,7 @@ def ragged_bench(args):
             "workload": f"ragged {args.ncol} columns, K U{{10..60}}, depth U[0.1,0.5] m, adaptive EMP tol 1e-6, "
                         f"dt in [1,3600] s; {args.warmup} warm-up day(s), then {args.span_hours} h timed",
             "cells": int(dom.ncell), "seed": args.seed, "arith": "fast" if args.arith else "reference",
+            "emp_weights": "geometric" if args.emp_weights else "product",
         },
         "best": {"schedule": best_row["schedule"], "shape": best_row["shape"]},
         "speedup_vs_launch_per_attempt": best_row["column_steps_per_s"] / step_row["column_steps_per_s"],
@@ -186,6 +188,8 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--arith", type=int, default=1, help="0 reference operation order, 1 fast form")
+    ap.add_argument("--emp-weights", type=int, default=0,
+                    help="Patankar factor of adaptive EMP: 0 product (default), 1 geometric mean")
     ap.add_argument("--workload", choices=("shelf", "ragged"), default="shelf",
                     help="shelf: the benchmark line; ragged: schedule/autotuner comparison on ragged columns")
     ap.add_argument("--ncol", type=int, default=100000, help="ragged: columns")
@@ -212,6 +216,7 @@ def main(argv=None):
     t = time.perf_counter()
     dom = cases.shelf_sea(n=args.n, days=days, seed=args.seed)
     dom.params.arith = args.arith
+    dom.params.emp_weights = args.emp_weights
     gen_s = time.perf_counter() - t
     wet = dom.mask == 0
     K = dom.nlayers
@@ -239,14 +244,14 @@ def main(argv=None):
     newt = (d1["newton"] - d0["newton"])[wet]
     colsteps = int(acc.sum())
     value = colsteps / wall
-    fl = flops.emp_flops(K[wet], acc + rej, newt)
+    fl = flops.emp_flops(K[wet], acc + rej, newt, weights=args.emp_weights)
     avg_launch_s = kernel_ms * 1e-3 / max(launches, 1)
     achieved = fl / max(launches, 1) / avg_launch_s / 1e12
     xs = g.get_state()
     land_cells = np.repeat(dom.mask, K) != 0
     min_state = float(xs[:, ~land_cells].min())
     g.close()
-    default_workload = args.n == 300 and args.seed == 0 and args.arith == 1
+    default_workload = args.n == 300 and args.seed == 0 and args.arith == 1 and args.emp_weights == 0
     traffic = DEFAULT_TRAFFIC_BYTES if default_workload else None
 
     out = {
@@ -271,6 +276,7 @@ def main(argv=None):
             "timed_days": args.steps,
             "seed": args.seed,
             "arith": "fast" if args.arith else "reference",
+            "emp_weights": "geometric" if args.emp_weights else "product",
         },
         "roofline": {
             "bound": "mfma",
@@ -287,6 +293,7 @@ def main(argv=None):
             "avg_launch_ms": avg_launch_s * 1e3,
         },
         "accepted_column_steps": colsteps,
+        "simulated_column_days_per_s": float(wet.sum()) * args.steps / wall,
         "rejected_attempts": int(rej.sum()),
         "newton_iterations": int(newt.sum()),
         "min_state": min_state,

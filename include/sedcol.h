@@ -119,7 +119,11 @@ typedef struct {
   int32_t clamp;              /* clamp to the minimum 0 after each fixed step, as
                                  fabm_sediment_component.F90:1726-1731 does */
   int32_t arith;              /* enum sedcol_arith */
-  int32_t reserved;           /* must be 0 */
+  int32_t emp_weights;        /* Patankar factor of the EMP steppers: 0 the
+                                 product over the destroyed components
+                                 (Bruggeman et al. 2007), 1 their geometric
+                                 mean (the same properties; long columns are
+                                 not throttled by their component count) */
 } sedcol_params;
 
 /* Per-column diagnostics (sedcol_get_diagnostics), the per-column form of

@@ -606,7 +606,7 @@ int sedcol_set_params(sedcol_handle *h, const sedcol_params *p)
   if (!(p->emp_rtol >= 0.0) || !(p->emp_atol > 0.0)) return fail(SEDCOL_EINVAL, "need emp_rtol >= 0, emp_atol > 0");
   if (p->arith != SEDCOL_ARITH_REFERENCE && p->arith != SEDCOL_ARITH_FAST)
     return fail(SEDCOL_EINVAL, "arith must be SEDCOL_ARITH_REFERENCE or SEDCOL_ARITH_FAST");
-  if (p->reserved != 0) return fail(SEDCOL_EINVAL, "reserved must be 0");
+  if (p->emp_weights != 0 && p->emp_weights != 1) return fail(SEDCOL_EINVAL, "emp_weights must be 0 or 1");
   h->p = *p;
   h->user_arith = p->arith;
   if (h->zw_on) h->p.arith = sc::ARITH_EXT;

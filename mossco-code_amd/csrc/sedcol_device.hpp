@@ -514,4 +514,94 @@ __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g
   return (!STAGE2 && pp > 1.0) ? 1.0 : pp;
 }
 
+// Patankar factor with geometric-mean weights: p = (S prod_j (1 + a_j p))^(1/N)
+// over the N destroyed components of the column, with the a_j and S of
+// psolve.  The product form's p falls below 1 by about the sum of the |a_j|,
+// which grows with the number of components (layers x variables) and
+// throttles long columns; here it falls by about their mean.  The scheme's
+// properties stay: one factor scales every rate (conservation), the root
+// lies below 1/|a_j| for every j (positivity), and p = 1 + O(h) in the
+// first stage and 1 + O(h^2) in the second, as in the product form.
+// With q = 1/p = qmin + d, qmin = max(-a_j) and b_j = qmin + a_j >= 0, the
+// root solves L = ln S + sum_j ln(b_j + d) = 0.  L is convex and increasing
+// in w = ln d, so Newton in w from d0 = S^(-1/N) (L >= 0) decreases d
+// monotonically and quadratically to the root, every iterate giving
+// p <= p*, also when the root lies next to qmin (a component driven almost
+// to zero in one step, d*/qmin far below rounding).  Per lane the factors
+// (b_j + d) s, s = 1/(qmin + d0) <= 1, are multiplied with their
+// derivative; L and dL/dw = d sum 1/(b_j + d) come from one log and one
+// division per lane.
+// The b_j live in LDS (bs[j * RS], the lane's slots): they are read once per
+// iteration and would otherwise hold 22 VGPRs through the loop.
+template <int LPC, bool STAGE2, int RS>
+__device__ __forceinline__ double psolve_geo(const double (&c)[NV], const double (&g)[NV], const double (&den)[NV],
+                                             double h, bool active, int64_t &zero_sink, int64_t &newton,
+                                             double *bs)
+{
+  double b[NV];
+  double Sl = 1.0, nl = 0.0, am = 0.0;
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    double av = 1.0;  // > 0 marks a component that is not destroyed
+    if (active && g[n] < 0.0) {
+      if (c[n] > 0.0) {
+        av = (h * g[n]) / c[n];
+        nl = nl + 1.0;
+        if (0.0 - av > am) am = 0.0 - av;
+        if (STAGE2 && den[n] > 0.0) Sl = Sl * (c[n] / den[n]);
+      } else {
+        zero_sink += 1;
+      }
+    }
+    b[n] = av;
+  }
+  const double N = lg_sum<LPC>(nl);
+  if (!(N > 0.0)) return 1.0;
+  const double lnS = STAGE2 ? lg_sum<LPC>(active ? dlog(Sl) : 0.0) : 0.0;
+  const double qmin = lg_max<LPC>(am);
+#pragma unroll
+  for (int n = 0; n < NV; ++n) bs[n * RS] = b[n] > 0.0 ? -1.0 : qmin + b[n];  // b < 0: not destroyed
+  double d = STAGE2 ? dexp((0.0 - lnS) / N) : 1.0;
+  const double sc = 1.0 / (qmin + d);
+  const double lsc = dlog(sc);
+  for (int it = 0; it < 60; ++it) {
+    newton += 1;
+    // the slots are re-read in every iteration, not held in registers
+    __asm__ volatile("" ::: "memory");
+    double P = 1.0, D = 0.0;
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      const double bn = bs[n * RS];
+      const bool on = bn >= 0.0;
+      const double f = on ? (bn + d) * sc : 1.0;
+      D = fma(D, f, on ? P * sc : 0.0);
+      P = P * f;
+    }
+    const double Ls = lg_sum<LPC>(active ? dlog(P) : 0.0);
+    const double Rs = lg_sum<LPC>(active ? D / P : 0.0);
+    const double L = (lnS + Ls) - N * lsc;
+    const double Lw = d * Rs;
+    const double dn = d * dexp(0.0 - L / Lw);
+    if (!(dn > 0.0) || !(dn < 1.0e300)) break;  // NaN, underflow or overflow: keep d
+    if (it > 0 && !(dn < d)) break;              // decreasing after the first step
+    const double step = d - dn;
+    const bool conv = step <= 0x1p-20 * dn || step <= 0x1p-40 * (qmin + dn);
+    d = dn;
+    if (conv) break;
+  }
+  const double pg = 1.0 / (qmin + d);
+  return (!STAGE2 && pg > 1.0) ? 1.0 : pg;
+}
+
+// The Patankar factor of a weighting (sedcol_params.emp_weights), fixed per
+// kernel instance: PW = 0 the product over the destroyed components, PW = 1
+// their geometric mean.
+template <int LPC, bool STAGE2, int PW, int RS>
+__device__ __forceinline__ double pfactor(const double (&c)[NV], const double (&g)[NV], const double (&den)[NV],
+                                          double h, bool active, int64_t &zero_sink, int64_t &newton, double *bs)
+{
+  if constexpr (PW == 1) return psolve_geo<LPC, STAGE2, RS>(c, g, den, h, active, zero_sink, newton, bs);
+  else return psolve<LPC, STAGE2>(c, g, den, h, active, zero_sink, newton);
+}
+
 }  // namespace sc

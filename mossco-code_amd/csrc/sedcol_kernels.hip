@@ -100,13 +100,15 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(Dev d, const int *list, int nlist
 // fixed-step window: EULER, RK4, RK4 3/8, EMP-1, EMP-2
 // ---------------------------------------------------------------------------
 
-template <int LPC, int METHOD, int ARITH>
+template <int LPC, int METHOD, int ARITH, int PW = 0>
 __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nlist, sedcol_params p, int rec,
                                                  double dt, int nsteps, int *errflag)
 {
   constexpr int G = BLOCK / LPC;
   __shared__ double sbc[G][2 * NV];
   __shared__ double sF[G][4][NV];
+  // coefficients of the geometric-mean Patankar solve (one slot otherwise)
+  __shared__ double sgeo[PW == 1 ? NV : 1][BLOCK];
   const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
   const int slot = blockIdx.x * G + g;
   const bool valid = slot < nlist;
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
         f0[n] = r;
         if (l == 0) F[0][n] = Fv;
       }, &d.zw);
-      const double p1 = psolve<LPC, false>(x, f0, x, dt, act, zs, dg.newton);
+      const double p1 = pfactor<LPC, false, PW, BLOCK>(x, f0, x, dt, act, zs, dg.newton, &sgeo[0][threadIdx.x]);
       if (p1 < dg.p_min) dg.p_min = p1;
       if (METHOD == SEDCOL_EMP1) {
 #pragma unroll
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
           f0[n] = (f0[n] + r) * 0.5;
           if (l == 0) F[1][n] = Fv;
         }, &d.zw);
-        const double p2 = psolve<LPC, true>(x, f0, c1, dt, act, zs, dg.newton);
+        const double p2 = pfactor<LPC, true, PW, BLOCK>(x, f0, c1, dt, act, zs, dg.newton, &sgeo[0][threadIdx.x]);
         if (p2 < dg.p_min) dg.p_min = p2;
 #pragma unroll
         for (int n = 0; n < NV; ++n) x[n] = x[n] + (dt * p2) * f0[n];
@@ -348,7 +350,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixed(Dev d, const int *list, int nli
 // is past r1 are skipped.  A column's operations are the same however its
 // attempts are split over launches, so windowed and single-launch runs give
 // identical results.
-template <int LPC, int ARITH, int BLK>
+template <int LPC, int ARITH, int BLK, int PW>
 __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
     const Dev *dptr, const int *list, int nlist, const sedcol_params *__restrict__ pptr,
     const EmpCtl *__restrict__ cptr, int *next, int *pending, int *work, int *errflag)
@@ -379,6 +381,8 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
   // touched only when an attempt is accepted, so they live in LDS and leave
   // the registers to the right-hand side.
   __shared__ double sacc[7][BLK];
+  // coefficients of the geometric-mean Patankar solve (one slot otherwise)
+  __shared__ double sgeo[PW == 1 ? NV : 1][BLK];
   double &sinkC = sacc[0][threadIdx.x], &sinkN = sacc[1][threadIdx.x], &fint = sacc[2][threadIdx.x];
   double &sinkCc = sacc[3][threadIdx.x], &sinkNc = sacc[4][threadIdx.x], &fintc = sacc[5][threadIdx.x];
   double &flast = sacc[6][threadIdx.x];
@@ -468,7 +472,7 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
         f0[n] = rr;
         if (l == 0) F[0][n] = Fv;
       }, &d.zw);
-      const double p1 = psolve<LPC, false>(x, f0, x, h, act, zs, dg.newton);
+      const double p1 = pfactor<LPC, false, PW, BLK>(x, f0, x, h, act, zs, dg.newton, &sgeo[0][threadIdx.x]);
 #pragma unroll
       for (int n = 0; n < NV; ++n) {
         const double y = x[n] + (h * p1) * f0[n];
@@ -478,7 +482,7 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
         f0[n] = (f0[n] + rr) * 0.5;
         if (l == 0) F[1][n] = Fv;
       }, &d.zw);
-      const double p2 = psolve<LPC, true>(x, f0, c1, h, act, zs, dg.newton);
+      const double p2 = pfactor<LPC, true, PW, BLK>(x, f0, c1, h, act, zs, dg.newton, &sgeo[0][threadIdx.x]);
       // err = max |c2 - c1| / sc: the lane's largest ratio by
       // cross-multiplication over the variables, one division, then the
       // maximum over the layers
@@ -863,21 +867,35 @@ void launch_rhs(int lpc, hipStream_t st, const Dev &d, const int *list, int n, c
 void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *list, int n,
                   const sedcol_params &p, int rec, double dt, int nsteps, int *errflag)
 {
-#define SC_FIX(M)                                                                                                   \
-  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_fixed<LPC, M, AR>), grid_for<LPC>(n), dim3(BLOCK), \
-                                                               0, st, d, list, n, p, rec, dt, nsteps, errflag)))
+#define SC_FIX(M, W)                                                                                             \
+  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_fixed<LPC, M, AR, W>), grid_for<LPC>(n),        \
+                                                               dim3(BLOCK), 0, st, d, list, n, p, rec, dt, nsteps, \
+                                                               errflag)))
+  const bool geo = p.emp_weights == 1;
   switch (method) {
-    case SEDCOL_EULER: SC_FIX(SEDCOL_EULER); break;
-    case SEDCOL_RUNGE_KUTTA_4: SC_FIX(SEDCOL_RUNGE_KUTTA_4); break;
-    case SEDCOL_RUNGE_KUTTA_4_38: SC_FIX(SEDCOL_RUNGE_KUTTA_4_38); break;
-    case SEDCOL_EMP1: SC_FIX(SEDCOL_EMP1); break;
-    default: SC_FIX(SEDCOL_EMP2); break;
+    case SEDCOL_EULER: SC_FIX(SEDCOL_EULER, 0); break;
+    case SEDCOL_RUNGE_KUTTA_4: SC_FIX(SEDCOL_RUNGE_KUTTA_4, 0); break;
+    case SEDCOL_RUNGE_KUTTA_4_38: SC_FIX(SEDCOL_RUNGE_KUTTA_4_38, 0); break;
+    case SEDCOL_EMP1:
+      if (geo) {
+        SC_FIX(SEDCOL_EMP1, 1);
+      } else {
+        SC_FIX(SEDCOL_EMP1, 0);
+      }
+      break;
+    default:
+      if (geo) {
+        SC_FIX(SEDCOL_EMP2, 1);
+      } else {
+        SC_FIX(SEDCOL_EMP2, 0);
+      }
+      break;
   }
 #undef SC_FIX
 }
 
 // blocks of a work-queue kernel that are resident at once on this device
-template <int LPC, int AR, int BLK>
+template <int LPC, int AR, int BLK, int PW>
 static int emp_resident()
 {
   static int cached = 0;
@@ -885,7 +903,7 @@ static int emp_resident()
     int dev = 0, per_cu = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive<LPC, AR, BLK>, BLK, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive<LPC, AR, BLK, PW>, BLK, 0) != hipSuccess)
       per_cu = 1;
     cached = (per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount;
   }
@@ -899,6 +917,11 @@ static int emp_resident()
     case 256: { constexpr int BLK = 256; __VA_ARGS__; } break;   \
     default: return false;                                       \
   }
+#define SC_DISPATCH_PW(p, ...)                                   \
+  switch ((p).emp_weights) {                                     \
+    case 1: { constexpr int PW = 1; __VA_ARGS__; } break;        \
+    default: { constexpr int PW = 0; __VA_ARGS__; } break;       \
+  }
 
 static bool emp_shape_ok(int lpc, int blk)
 {
@@ -910,7 +933,7 @@ int emp_resident_blocks(int lpc, int blk, const sedcol_params &p)
   if (!emp_shape_ok(lpc, blk)) return 0;
   int out = 0;
   auto body = [&]() -> bool {
-    SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, out = emp_resident<LPC, AR, BLK>())));
+    SC_DISPATCH_PW(p, SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, out = emp_resident<LPC, AR, BLK, PW>()))));
     return true;
   };
   body();
@@ -922,13 +945,13 @@ bool launch_emp_adaptive(int lpc, int blk, hipStream_t st, const Dev *d_dev, con
                          int *pending, int *work, int *errflag)
 {
   if (!emp_shape_ok(lpc, blk)) return false;
-  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, {
+  SC_DISPATCH_PW(p, SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, {
     const int G = BLK / LPC;
     const unsigned need = (unsigned)((n + G - 1) / G);
-    const unsigned cap = (unsigned)emp_resident<LPC, AR, BLK>();
-    hipLaunchKernelGGL((k_emp_adaptive<LPC, AR, BLK>), dim3(need < cap ? need : cap), dim3(BLK), 0, st, d_dev,
+    const unsigned cap = (unsigned)emp_resident<LPC, AR, BLK, PW>();
+    hipLaunchKernelGGL((k_emp_adaptive<LPC, AR, BLK, PW>), dim3(need < cap ? need : cap), dim3(BLK), 0, st, d_dev,
                        list, n, d_params, d_ctl, next, pending, work, errflag);
-  })));
+  }))));
   return true;
 }
 

@@ -41,7 +41,7 @@ module sedcol_hip
     real(c_double) :: rPads = (0.01d0*20.d0)/86400.d0, PAdsODU = 70.d0
     real(c_double) :: emp_rtol = 1.d-6, emp_atol = 1.d-6, emp_dt_min = 1.d0, emp_dt_max = 3600.d0
     integer(c_int32_t) :: bcup_dis = 2, bcup_part = 1, bcdown = 3, clamp = 1
-    integer(c_int32_t) :: arith = 0, reserved = 0
+    integer(c_int32_t) :: arith = 0, emp_weights = 0
   end type
 
   interface

@@ -38,6 +38,9 @@ TRANSPORT_DIS = 7
 RHS_PER_CELL = KINETICS + N_PART * TRANSPORT_PART + N_DIS * TRANSPORT_DIS  # 173
 EMP_ATTEMPT_PER_CELL = 2 * RHS_PER_CELL + 2 * NV * 2 + NV * 2 + NV * 2 + NV * 2 + NV * 6 + (NV * 2 + 4)
 NEWTON_PER_CELL = NV * 6 + 4
+#: geometric-mean weights: the scaled factors and their derivative, a log
+#: (13-term series, ~30 operations), the derivative ratio and two sums
+NEWTON_GEO_PER_CELL = NV * 7 + 36
 RK4_STEP_PER_CELL = 4 * RHS_PER_CELL + 3 * NV * 2 + NV * 4 + 12
 
 #: HBM bytes a persistent adaptive launch moves per cell and per column,
@@ -52,11 +55,13 @@ BYTES_PER_COLUMN_LAUNCH = 8 + 8 + 2 * 3 * NV * 8 + 2 * 88
 BYTES_PER_COLUMN_RECORD = 8 + 2 * NV * 8
 
 
-def emp_flops(cells_per_col, attempts, newton):
-    """FLOPs of adaptive-EMP work: attempts and Newton iterations per column."""
+def emp_flops(cells_per_col, attempts, newton, weights=0):
+    """FLOPs of adaptive-EMP work: attempts and Newton iterations per column
+    (weights: the Patankar weighting, 0 product, 1 geometric mean)."""
     K = np.asarray(cells_per_col, dtype=np.float64)
+    per_newton = NEWTON_GEO_PER_CELL if weights == 1 else NEWTON_PER_CELL
     return float(np.sum(K * (EMP_ATTEMPT_PER_CELL * np.asarray(attempts, dtype=np.float64)
-                             + NEWTON_PER_CELL * np.asarray(newton, dtype=np.float64))))
+                             + per_newton * np.asarray(newton, dtype=np.float64))))
 
 
 def rk4_flops(cells_per_col, steps):

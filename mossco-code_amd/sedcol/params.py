@@ -71,6 +71,7 @@ class SedParams:
     bcdown: int = 3
     clamp: int = 1
     arith: int = 0                     # 0 reference operation order, 1 fast form
+    emp_weights: int = 0               # Patankar factor: 0 product, 1 geometric mean
 
     def si(self) -> dict:
         """Flat dict in the field order of sedcol_params / oc_params."""
@@ -105,7 +106,7 @@ class SedParams:
             "bcdown": int(self.bcdown),
             "clamp": int(self.clamp),
             "arith": int(self.arith),
-            "reserved": 0,
+            "emp_weights": int(self.emp_weights),
         }
 
 
@@ -120,7 +121,7 @@ PARAM_FIELDS = [
     ("emp_dt_min", ctypes.c_double), ("emp_dt_max", ctypes.c_double),
     ("bcup_dis", ctypes.c_int32), ("bcup_part", ctypes.c_int32),
     ("bcdown", ctypes.c_int32), ("clamp", ctypes.c_int32),
-    ("arith", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ("arith", ctypes.c_int32), ("emp_weights", ctypes.c_int32),
 ]
 
 

@@ -281,7 +281,7 @@ def predict(model: Model, X, net: Network | None = None):
 
 
 def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_tol=1.0e-6, flux_rtol=0.01,
-             tails=0.05, log=None):
+             tails=0.05, emp_weights=0, log=None):
     """Surrogate warm start against cold spin-up on n drawn columns (layers
     U{20..40}, a fraction `tails` of the inputs outside the training ranges).
 
@@ -295,7 +295,9 @@ def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_
           to the cold spin-up.
     The wall times (warm including prediction and fallback) and the
     integration work (attempts x layers) of both are reported, with the
-    warm start's errors per variable.  Returns a dict."""
+    warm start's errors per variable.  emp_weights selects the Patankar
+    factor of every integration (0 product, 1 geometric mean).  Returns a
+    dict."""
     import time
 
     from . import capi, spinup
@@ -304,6 +306,7 @@ def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_
     s = spinup.draw_inputs(rng, n, tails=tails)
     K = rng.integers(20, 41, size=n)
     dom = spinup.build(s, K)
+    dom.params.emp_weights = emp_weights
     floor = spinup.flux_floor(s)
     say = log or (lambda m: None)
 
@@ -356,6 +359,7 @@ def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_
     tail = s["tail"]
     return {
         "columns": n, "tail_fraction": float(tail.mean()), "years": years, "warm_days": warm_days,
+        "emp_weights": "geometric" if emp_weights else "product",
         "res_tol": res_tol, "flux_rtol": flux_rtol,
         "wall_cold_s": cold["wall_s"], "wall_warm_s": wall_warm, "speedup_wall": cold["wall_s"] / wall_warm,
         "wall_predict_s": t_pred, "predict_kernel_ms": pred_ms, "wall_warm_integration_s": warm["wall_s"],
@@ -398,6 +402,7 @@ def _main(argv=None):
     e.add_argument("--seed", type=int, default=7)
     e.add_argument("--years", type=float, default=2.0)
     e.add_argument("--warm-days", type=float, default=30.0)
+    e.add_argument("--emp-weights", type=int, default=0, help="Patankar factor: 0 product, 1 geometric mean")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "train":
@@ -434,7 +439,7 @@ def _main(argv=None):
                           "predict_tflops": flops_per_sample() * a.rows / (pms * 1e-3) / 1e12}), flush=True)
     else:
         model = Model.load(a.model)
-        r = evaluate(model, a.n, a.seed, years=a.years, warm_days=a.warm_days, log=log)
+        r = evaluate(model, a.n, a.seed, years=a.years, warm_days=a.warm_days, emp_weights=a.emp_weights, log=log)
         print(json.dumps(r), flush=True)
 
 

@@ -29,7 +29,7 @@ _PARAM_FIELDS = [
     ("emp_dt_min", ctypes.c_double), ("emp_dt_max", ctypes.c_double),
     ("bcup_dis", ctypes.c_int32), ("bcup_part", ctypes.c_int32),
     ("bcdown", ctypes.c_int32), ("clamp", ctypes.c_int32),
-    ("arith", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ("arith", ctypes.c_int32), ("emp_weights", ctypes.c_int32),
 ]
 
 

@@ -533,10 +533,95 @@ static double col_psolve2(const double *c, long ks, const double *g, long gs,
   return pp;
 }
 
-static double col_psolve(const double *c, long ks, const double *g, long gs,
-                         double dt, int K, int64_t *zero_sink, int64_t *newton)
+/* Geometric-mean weights (sedcol_params.emp_weights = 1): the root p of
+ * p = (S prod_{j: g_j<0} (1 + a_j p))^(1/N) over the N destroyed components,
+ * with the a_j and S of col_psolve2.  One factor for all rates keeps mass
+ * conservation, the root below every 1/|a_j| keeps positivity, and p - 1 is
+ * O(dt) in the first stage and O(dt^2) in the second, as for the product;
+ * but p falls below 1 by about the mean of the |a_j| rather than their sum,
+ * so long columns are not throttled by their number of components.
+ * q = 1/p = qmin + d with qmin = max(-a_j), b_j = qmin + a_j >= 0: the root
+ * solves L = ln S + sum_j ln(b_j + d) = 0, convex and increasing in ln d, so
+ * Newton in ln d from d0 = S^(-1/N) (L >= 0) decreases d monotonically and
+ * quadratically to the root with p <= p* throughout, also when the root lies
+ * next to qmin.  Per cell the scaled factors (b_j + d)/(qmin + d0) and their
+ * derivative; one log and one division per cell and iteration. */
+static double col_psolve_geo(const double *c, long ks, const double *g, long gs,
+                             const double *den, long dens, double dt, int K,
+                             int64_t *zero_sink, int64_t *newton)
 {
-  return col_psolve2(c, ks, g, gs, NULL, 0, dt, K, zero_sink, newton);
+  double b[NV][OC_KMAX], Sl[OC_KMAX], nl[OC_KMAX], am[OC_KMAX], t[OC_KMAX];
+  for (int k = 0; k < K; ++k) { Sl[k] = 1.0; nl[k] = 0.0; am[k] = 0.0; }
+  for (int n = 0; n < NV; ++n)
+    for (int k = 0; k < K; ++k) {
+      double gv = g[(long)n * gs + k], cv = c[(long)n * ks + k];
+      double av = 1.0; /* > 0: not destroyed */
+      if (gv < 0.0) {
+        if (cv > 0.0) {
+          av = (dt * gv) / cv;
+          nl[k] = nl[k] + 1.0;
+          if (0.0 - av > am[k]) am[k] = 0.0 - av;
+          if (den) {
+            double dv = den[(long)n * dens + k];
+            if (dv > 0.0) Sl[k] = Sl[k] * (cv / dv);
+          }
+        } else {
+          ++*zero_sink;
+        }
+      }
+      b[n][k] = av;
+    }
+  const double N = tree_sum(nl, K);
+  if (!(N > 0.0)) return 1.0;
+  double lnS = 0.0;
+  if (den) {
+    for (int k = 0; k < K; ++k) t[k] = oc_log(Sl[k]);
+    lnS = tree_sum(t, K);
+  }
+  double qmin = 0.0;
+  for (int k = 0; k < K; ++k)
+    if (am[k] > qmin) qmin = am[k];
+  for (int n = 0; n < NV; ++n)
+    for (int k = 0; k < K; ++k) b[n][k] = b[n][k] > 0.0 ? -1.0 : qmin + b[n][k];
+  double d = den ? oc_exp((0.0 - lnS) / N) : 1.0;
+  const double sc = 1.0 / (qmin + d);
+  const double lsc = oc_log(sc);
+  double lp[OC_KMAX], r[OC_KMAX];
+  for (int it = 0; it < 60; ++it) {
+    ++*newton;
+    for (int k = 0; k < K; ++k) {
+      double P = 1.0, D = 0.0;
+      for (int n = 0; n < NV; ++n) {
+        const int on = b[n][k] >= 0.0;
+        const double f = on ? (b[n][k] + d) * sc : 1.0;
+        D = fma(D, f, on ? P * sc : 0.0);
+        P = P * f;
+      }
+      lp[k] = oc_log(P);
+      r[k] = D / P;
+    }
+    const double Ls = tree_sum(lp, K);
+    const double Rs = tree_sum(r, K);
+    const double L = (lnS + Ls) - N * lsc;
+    const double Lw = d * Rs;
+    const double dn = d * oc_exp(0.0 - L / Lw);
+    if (!(dn > 0.0) || !(dn < 1.0e300)) break;
+    if (it > 0 && !(dn < d)) break;
+    const double step = d - dn;
+    const int conv = step <= 0x1p-20 * dn || step <= 0x1p-40 * (qmin + dn);
+    d = dn;
+    if (conv) break;
+  }
+  const double pg = 1.0 / (qmin + d);
+  return (!den && pg > 1.0) ? 1.0 : pg;
+}
+
+static double col_pfactor(int weights, const double *c, long ks, const double *g, long gs,
+                          const double *den, long dens, double dt, int K,
+                          int64_t *zero_sink, int64_t *newton)
+{
+  if (weights == 1) return col_psolve_geo(c, ks, g, gs, den, dens, dt, K, zero_sink, newton);
+  return col_psolve2(c, ks, g, gs, den, dens, dt, K, zero_sink, newton);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -651,7 +736,7 @@ static int col_step(const oc_params *p, const colview *v, int method,
     memcpy(F1out, w->F[3], sizeof(double) * NV);
   } else { /* EMP-1 / EMP-2, fixed step */
     col_rhs(p, v, c, ks, w->r[0], S, w->F[0], w->s[0][0], w->s[0][1]);
-    double p1 = col_psolve(c, ks, w->r[0], S, dt, K, &dg->zero_sink, &dg->newton);
+    double p1 = col_pfactor(p->emp_weights, c, ks, w->r[0], S, NULL, 0, dt, K, &dg->zero_sink, &dg->newton);
     if (p1 < dg->p_min) dg->p_min = p1;
     if (method == OC_EMP1) {
       for (int n = 0; n < NV; ++n)
@@ -673,7 +758,7 @@ static int col_step(const oc_params *p, const colview *v, int method,
       for (int n = 0; n < NV; ++n)
         for (int k = 0; k < K; ++k)
           w->r[2][n * S + k] = (w->r[0][n * S + k] + w->r[1][n * S + k]) * 0.5;
-      double p2 = col_psolve2(c, ks, w->r[2], S, c1, S, dt, K, &dg->zero_sink, &dg->newton);
+      double p2 = col_pfactor(p->emp_weights, c, ks, w->r[2], S, c1, S, dt, K, &dg->zero_sink, &dg->newton);
       if (p2 < dg->p_min) dg->p_min = p2;
       for (int n = 0; n < NV; ++n)
         for (int k = 0; k < K; ++k) c[n * ks + k] = c[n * ks + k] + (dt * p2) * w->r[2][n * S + k];
@@ -727,7 +812,7 @@ static int col_emp_adaptive(const oc_params *p, const colview *v, double *c,
     int clipped = !(dt < rem);
     double h = clipped ? rem : dt;
     col_rhs(p, v, c, ks, w->r[0], S, w->F[0], w->s[0][0], w->s[0][1]);
-    double p1 = col_psolve(c, ks, w->r[0], S, h, K, &dg->zero_sink, &dg->newton);
+    double p1 = col_pfactor(p->emp_weights, c, ks, w->r[0], S, NULL, 0, h, K, &dg->zero_sink, &dg->newton);
     for (int n = 0; n < NV; ++n)
       for (int k = 0; k < K; ++k) {
         double y = c[n * ks + k] + (h * p1) * w->r[0][n * S + k];
@@ -737,7 +822,7 @@ static int col_emp_adaptive(const oc_params *p, const colview *v, double *c,
     for (int n = 0; n < NV; ++n)
       for (int k = 0; k < K; ++k)
         g[n * S + k] = (w->r[0][n * S + k] + w->r[1][n * S + k]) * 0.5;
-    double p2 = col_psolve2(c, ks, g, S, c1, S, h, K, &dg->zero_sink, &dg->newton);
+    double p2 = col_pfactor(p->emp_weights, c, ks, g, S, c1, S, h, K, &dg->zero_sink, &dg->newton);
     /* err = max_j |c2_j - c1_j| / sc_j: per layer the largest ratio is found
      * by cross-multiplication over the variables (in order) and divided
      * once, then the maximum is taken over the layers */

@@ -63,7 +63,7 @@ typedef struct {
   int32_t bcdown;       /* 1 flux(0), 2 Dirichlet(0), 3 zero-gradient */
   int32_t clamp;        /* clamp to minimum (0) after every step */
   int32_t arith;        /* 0 reference operation order, 1 fast form */
-  int32_t reserved;
+  int32_t emp_weights;  /* Patankar factor: 0 product, 1 geometric mean */
 } oc_params;
 
 /* Bioturbation profile 3 (Zhang & Wirtz 2017; fabm_sediment_driver.F90:

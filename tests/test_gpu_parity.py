@@ -105,6 +105,29 @@ def test_rk4_30_days(gpu_ready, arith):
         assert r.max() <= 1e-12
 
 
+@pytest.mark.parametrize("arith", ARITHS)
+@pytest.mark.parametrize("method", [EMP1, EMP2])
+def test_geometric_weights_one_step(gpu_ready, method, arith):
+    """Fixed EMP steps with geometric-mean Patankar weights, bitwise."""
+    dom = cases.plain_columns(seed=1)
+    dom.params.arith = arith
+    dom.params.emp_weights = 1
+    p = dom.params.si()
+    x0 = _perturbed(dom, seed=4)
+    for dt in (360.0, 86400.0):
+        xg = x0.copy()
+        with capi.SedimentColumns.from_domain(dom) as g:
+            fg = g.ode_solver(xg, dt, method)
+            dg = g.diagnostics()
+        xo = x0.copy()
+        orc = O.Oracle(dom, p)
+        assert orc.ode_solver(xo, dt, method) == 0
+        assert np.array_equal(xg, xo)
+        assert np.array_equal(fg, orc.flux)
+        assert np.array_equal(dg["newton"], orc.diag["newton"])
+        assert np.all(xg >= 0.0)
+
+
 def test_rk4_windows_equal_single_launch(gpu_ready):
     dom = cases.plain_columns(ni=2, nj=2, seed=2)
     with capi.SedimentColumns.from_domain(dom) as g:
@@ -175,11 +198,13 @@ def test_emp_adaptive_shelf_subset(gpu_ready, arith):
         assert r.max() <= 1e-12, (e, r.max())
 
 
+@pytest.mark.parametrize("weights", [pytest.param(0, id="product"), pytest.param(1, id="geometric")])
 @pytest.mark.parametrize("arith", ARITHS)
-def test_emp_ragged(gpu_ready, arith):
+def test_emp_ragged(gpu_ready, arith, weights):
     dom = cases.ragged_columns(ncol=96, seed=4)
     assert set(np.unique(np.where(dom.nlayers <= 16, 16, np.where(dom.nlayers <= 32, 32, 64)))) == {16, 32, 64}
     dom.params.arith = arith
+    dom.params.emp_weights = weights
     p = dom.params.si()
     span = 6 * 3600.0
     with capi.SedimentColumns.from_domain(dom) as g:
@@ -204,10 +229,12 @@ def _edge_columns():
     return cases.build_columns(K, np.linspace(0.05, 0.5, K.size), s, params=p)
 
 
+@pytest.mark.parametrize("weights", [pytest.param(0, id="product"), pytest.param(1, id="geometric")])
 @pytest.mark.parametrize("arith", ARITHS)
-def test_layer_count_edges(gpu_ready, arith):
+def test_layer_count_edges(gpu_ready, arith, weights):
     dom = _edge_columns()
     dom.params.arith = arith
+    dom.params.emp_weights = weights
     p = dom.params.si()
     # RK4, one simulated day in one launch
     with capi.SedimentColumns.from_domain(dom) as g:

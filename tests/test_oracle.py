@@ -15,7 +15,9 @@ What pins what:
     with it is unpinned) -- C/N/P stoichiometry, oxic/anoxic limits, zero
     destruction of an exhausted species;
   * EMP -- the closed form c/(1 + k dt) of one linear sink, positivity at any
-    step, conservation, observed orders 1 and 2;
+    step, conservation, observed orders 1 and 2, for both Patankar weightings;
+    the geometric-mean factor of both stages against a bracketing root
+    solve of its defining equation;
   * exp/rsqrt -- against libm.
 A planted mistake in any of these formulas fails at least one test here.
 """
@@ -254,22 +256,123 @@ def test_kinetics_exhausted_species_not_destroyed():
 # EMP
 # ---------------------------------------------------------------------------
 
-def test_emp1_single_linear_sink_closed_form():
+@pytest.mark.parametrize("weights", [0, 1])
+def test_emp1_single_linear_sink_closed_form(weights):
     d = _decay_domain(K=1, ncol=1)
+    d.params.emp_weights = weights
     p = d.params.si()
     k = O.oc_exp(-4500.0 * (1.0 / (d.temp[0, 0] + 273.0) - 1.0 / 288.0)) * p["rL"]
     for dt in (10.0, 1.0e5, 1.0e9):
         x = d.conc0.copy()
         O.Oracle(d, p).ode_solver(x, dt, EMP1)
         # c + dt p f cancels down to c/(1 + k dt): relative rounding grows
-        # with 1 + k dt
-        assert math.isclose(x[0, 0], d.conc0[0, 0] / (1.0 + k * dt), rel_tol=4e-16 * (1.0 + k * dt))
+        # with 1 + k dt (one destroyed component: both weightings solve the
+        # same equation; the geometric solve goes through exp and log)
+        tol = (4e-16 if weights == 0 else 1e-13) * (1.0 + k * dt)
+        assert math.isclose(x[0, 0], d.conc0[0, 0] / (1.0 + k * dt), rel_tol=tol)
+
+
+def _geo_factor(c, g, den, dt):
+    """Root of p = (S prod (1 + a_j p))^(1/N) by bracketing (independent of
+    the oracle's Newton iteration): a_j = dt g_j / c_j over g_j < 0, c_j > 0,
+    S = prod c_j / den_j over those with den_j > 0 (second stage)."""
+    m = (g < 0.0) & (c > 0.0)
+    a = dt * g[m] / c[m]
+    if a.size == 0:
+        return 1.0
+    lnS = 0.0
+    if den is not None:
+        dm = m & (den > 0.0)
+        lnS = float(np.sum(np.log(c[dm] / den[dm])))
+    N = a.size
+
+    def f(p):  # decreasing on (0, 1/max|a|), +inf at 0 and -inf at the end
+        with np.errstate(divide="ignore"):
+            return lnS + float(np.sum(np.log1p(a * p))) - N * math.log(p)
+
+    lo, hi = 1e-300, 1.0 / float(np.max(-a))
+    for _ in range(400):
+        mid = 0.5 * (lo + hi)
+        if mid in (lo, hi):
+            break
+        if f(mid) > 0.0:
+            lo = mid
+        else:
+            hi = mid
+    return lo
 
 
 @pytest.mark.parametrize("method", [EMP1, EMP2])
-def test_emp_positive_and_conservative_at_huge_steps(method):
+def test_emp_geometric_factor_against_root_solve(method):
+    """One EMP step with geometric-mean weights equals the step built from
+    bracketing roots of the factor equation (a per-column mean, not the
+    product, and S only in the second stage).  The product weighting gives
+    a different step, so a dropped 1/N fails here."""
+    d = cases.plain_columns(ni=1, nj=1, K=12, seed=21)
+    d.params.clamp = 0
+    d.params.emp_weights = 1
+    p = d.params.si()
+    orc = O.Oracle(d, p)
+    x0 = d.conc0.copy()
+    rng = np.random.default_rng(5)
+    x0[0:5] = rng.uniform(10.0, 1.0e3, size=x0[0:5].shape)
+    x0[8] = rng.uniform(0.0, 50.0, size=x0[8].shape)
+    x0[10] = rng.uniform(0.0, 10.0, size=x0[10].shape)
+    dt = 1800.0
+    f0, _ = orc.rhs(x0)
+    p1 = _geo_factor(x0, f0, None, dt)
+    c1 = np.maximum(x0 + (dt * p1) * f0, 0.0)
+    if method == EMP1:
+        expect = c1
+    else:
+        f1, _ = orc.rhs(c1)
+        g = (f0 + f1) * 0.5
+        p2 = _geo_factor(x0, g, c1, dt)
+        expect = np.maximum(x0 + (dt * p2) * g, 0.0)
+    x = x0.copy()
+    assert orc.ode_solver(x, dt, method) == 0
+    assert np.allclose(x, expect, rtol=1e-11, atol=1e-12)
+    # the product weighting's step differs
+    q = dataclasses.replace(d.params, emp_weights=0)
+    y = x0.copy()
+    assert O.Oracle(d, q.si()).ode_solver(y, dt, method) == 0
+    assert not np.allclose(y, expect, rtol=1e-6, atol=1e-9)
+
+
+def test_geometric_weights_hold_the_tolerance_on_a_long_column():
+    """Cold start of a 60-layer column, 10 minutes, adaptive EMP at tolerance
+    1e-6 with dt >= 1 s, against RK4 with dt = 0.01 s (which agrees with
+    either EMP form at tolerance 1e-10 to 1e-8).  Both forms are held at
+    dt_min by their error estimate, but the product factor throttles the
+    whole column by the sum of the relative destruction over its 660
+    components and ends percent-level wrong; the geometric mean stays at
+    the tolerance."""
+    dom = cases.ragged_columns(ncol=400, seed=4)
+    sub = dom.subset([71])
+    sub.params.arith = 1
+    assert sub.nlayers[0] == 60
+    x0 = sub.conc0.copy()
+    ref = x0.copy()
+    assert O.Oracle(sub, sub.params.si()).run(ref, 0.01, 60000, RUNGE_KUTTA_4) == 0
+    err = {}
+    for w in (0, 1):
+        q = dataclasses.replace(sub.params, emp_weights=w)
+        y = x0.copy()
+        o = O.Oracle(sub, q.si())
+        assert o.emp_integrate(y, 600.0) == 0
+        assert np.all(y >= 0.0)
+        err[w] = float(np.max(np.abs(y - ref) / (1e-6 + np.abs(ref))))
+    print("max rel error: product", err[0], "geometric", err[1])
+    assert err[1] <= 1e-5
+    assert err[0] >= 1e-3
+
+
+@pytest.mark.parametrize("weights", [0, 1])
+@pytest.mark.parametrize("method", [EMP1, EMP2])
+def test_emp_positive_and_conservative_at_huge_steps(method, weights):
     d = cases.plain_columns(ni=2, nj=2, K=15, seed=12)
     d.params.clamp = 0
+    d.params.emp_weights = weights
     p = d.params.si()
     x = d.conc0.copy()
     orc = O.Oracle(d, p)
@@ -282,12 +385,14 @@ def test_emp_positive_and_conservative_at_huge_steps(method):
         assert r.max() <= 1e-13, (e, r.max())
 
 
-def test_emp_observed_order():
+@pytest.mark.parametrize("weights", [0, 1])
+def test_emp_observed_order(weights):
     """Two linear decays in a closed one-layer column (no transport at all):
     EMP1 error ~ dt, EMP2 ~ dt^2."""
     d = _decay_domain(K=1, ncol=1)
     d.conc0[1] = d.conc0[0] * 0.5
     d.params.bcup_dis = 3
+    d.params.emp_weights = weights
     p = d.params.si()
     t_end = 40 * 86400.0
     fT = O.oc_exp(-4500.0 * (1.0 / (d.temp[0, 0] + 273.0) - 1.0 / 288.0))
@@ -306,9 +411,11 @@ def test_emp_observed_order():
     assert 1.8 < orders[EMP2] < 2.2, orders
 
 
-def test_emp_adaptive_budget_and_positivity():
+@pytest.mark.parametrize("weights", [0, 1])
+def test_emp_adaptive_budget_and_positivity(weights):
     full = cases.shelf_sea(n=30, days=2, seed=5)
     dom = full.subset(np.flatnonzero(full.mask == 0)[:24])
+    dom.params.emp_weights = weights
     p = dom.params.si()
     orc = O.Oracle(dom, p)
     x = dom.conc0.copy()
