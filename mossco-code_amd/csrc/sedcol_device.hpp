@@ -479,8 +479,9 @@ __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g
     a[n] = av;
   }
   const double Sc = STAGE2 ? lg_prod<LPC>(Sl) : 1.0;
-  // quadratic convergence from below; stop once a correction is below
-  // 2^-20 p (remaining error ~1e-12 p, positivity and conservation intact)
+  // quadratic convergence from below (positivity and conservation hold for
+  // every iterate); the stopping rule leaves a relative error near 1e-9 at
+  // most
   // First step, from p = 0: every factor is 1, so the product is 1 and its
   // derivative the plain sum of the a_j.  The butterfly sum below takes the
   // same operands in the same order as lg_pd would (x * 1 is exact), so the
@@ -493,6 +494,7 @@ __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g
   double pp = 0.0 - Sc / (Sc * D0 - 1.0);
   if (!(pp > 0.0)) return 0.0;
   if (pp <= 0x1p-20 * pp) return (!STAGE2 && pp > 1.0) ? 1.0 : pp;
+  double dprev = pp;  // the last correction
   for (int it = 1; it < 60; ++it) {
     newton += 1;
     double P = 1.0, D = 0.0;
@@ -507,7 +509,12 @@ __device__ __forceinline__ double psolve(const double (&c)[NV], const double (&g
     const double hp = Sc * D - 1.0;
     const double pn = pp - hv / hp;
     if (!(pn > pp)) break;
-    const bool conv = (pn - pp) <= 0x1p-20 * pn;
+    // Stop once the correction is below 2^-20 p, or once it is below
+    // 2^-10 p and the next one, dl^3 / dprev^2 by the quadratic
+    // convergence, would be below 2^-30 p.
+    const double dl = pn - pp;
+    const bool conv = dl <= 0x1p-20 * pn || (dl <= 0x1p-10 * pn && (dl * dl) * dl <= (0x1p-30 * pn) * (dprev * dprev));
+    dprev = dl;
     pp = pn;
     if (conv) break;
   }

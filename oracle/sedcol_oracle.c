@@ -504,9 +504,10 @@ static double col_psolve2(const double *c, long ks, const double *g, long gs,
   if (den) Sc = tree_prod(S, K);
   double pp = 0.0;
   double P[OC_KMAX], D[OC_KMAX];
-  /* Newton converges quadratically from below; it stops once a correction
-   * is below 2^-20 p (the remaining error is then ~1e-12 p: p below the root
-   * keeps every component positive, and c + dt p f conserves for any p). */
+  /* Newton converges quadratically from below (p below the root keeps
+   * every component positive, and c + dt p f conserves for any p); the
+   * stopping rule below leaves a relative error near 1e-9 at most. */
+  double dprev = INFINITY; /* the last correction */
   for (int it = 0; it < 60; ++it) {
     ++*newton;
     for (int k = 0; k < K; ++k) {
@@ -525,7 +526,13 @@ static double col_psolve2(const double *c, long ks, const double *g, long gs,
     double hp = Sc * D[0] - 1.0;
     double pn = pp - h / hp;
     if (!(pn > pp)) break;
-    int conv = (pn - pp) <= 0x1p-20 * pn;
+    /* stop once the correction is below 2^-20 p, or once it is below
+     * 2^-10 p and the next one, dl^3 / dprev^2 by the quadratic convergence
+     * (e_{k+1} = C e_k^2 with C estimated from the last two corrections),
+     * would be below 2^-30 p */
+    double dl = pn - pp;
+    int conv = dl <= 0x1p-20 * pn || (dl <= 0x1p-10 * pn && (dl * dl) * dl <= (0x1p-30 * pn) * (dprev * dprev));
+    dprev = dl;
     pp = pn;
     if (conv) break;
   }
