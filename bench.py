@@ -38,10 +38,10 @@ DAY = 86400.0
 
 #: HBM bytes per k_emp_adaptive launch of the default workload (300 x 300,
 #: seed 0, fast form), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over
-#: three timed days: mean FETCH_SIZE 421 005 KiB x 2.00 + WRITE_SIZE
-#: 272 317 KiB x 1.00, the factors being what the counters read for a plain
+#: three timed days: mean FETCH_SIZE 420 649 KiB x 2.00 + WRITE_SIZE
+#: 272 322 KiB x 1.00, the factors being what the counters read for a plain
 #: 256 MiB copy (FETCH_SIZE 131 084 KiB, WRITE_SIZE 262 144 KiB).
-DEFAULT_TRAFFIC_BYTES = (421005 * 2.0 + 272317 * 1.0) * 1024
+DEFAULT_TRAFFIC_BYTES = (420649 * 2.0 + 272322 * 1.0) * 1024
 
 
 def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads):
