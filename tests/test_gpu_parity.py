@@ -398,6 +398,37 @@ def test_bcup4_rhs_and_emp_bitwise(gpu_ready, arith):
     assert not np.array_equal(ro1, ro)
 
 
+def test_extensions_with_geometric_weights(gpu_ready):
+    """The kernel instances of the transport extensions (BcUp = 4, profile 3)
+    with geometric-mean Patankar weights: 6 hours of adaptive EMP each,
+    bitwise against the oracle."""
+    dom, fcap = _bcup4_domain()
+    dom.params.emp_weights = 1
+    p = dom.params.si()
+    with capi.SedimentColumns.from_domain(dom) as g:
+        g.set_flux_cap(fcap)
+        g.integrate(0.0, 6 * 3600.0)
+        xg = g.get_state()
+    xo = dom.conc0.copy()
+    assert O.Oracle(dom, p, fcap=fcap).emp_integrate(xo, 6 * 3600.0, nthreads=8) == 0
+    assert np.array_equal(xg, xo)
+    rng = np.random.Generator(np.random.Philox(14))
+    nl = np.array([2, 17, 33, 64] * 2, dtype=np.int32)
+    dom = cases.build_columns(nl, 0.2, cases.draw_scalars(rng, nl.size))
+    dom.params.emp_weights = 1
+    p = dom.params.si()
+    x = _perturbed(dom, seed=15)
+    with capi.SedimentColumns.from_domain(dom) as g:
+        g.set_bioturbation_zw()
+        g.set_state(x)
+        g.integrate(0.0, 6 * 3600.0)
+        xg = g.get_state()
+    xo = x.copy()
+    assert O.Oracle(dom, p, zw=capi.ZW_DEFAULTS).emp_integrate(xo, 6 * 3600.0, nthreads=8) == 0
+    assert np.array_equal(xg, xo)
+    assert np.all(xg >= 0.0)
+
+
 def test_bcup4_needs_flux_cap(gpu_ready):
     dom, fcap = _bcup4_domain()
     with capi.SedimentColumns.from_domain(dom) as g:
