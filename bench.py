@@ -209,7 +209,7 @@ def main(argv=None):
 
     import torch
 
-    from sedcol import cases, capi, flops
+    from sedcol import budget, cases, capi, flops
 
     torch.cuda.init()
     days = args.warmup + args.steps
@@ -221,6 +221,7 @@ def main(argv=None):
     wet = dom.mask == 0
     K = dom.nlayers
     g = capi.SedimentColumns.from_domain(dom)
+    inv_start = g.inventory()
     if args.warmup:
         g.integrate(0.0, args.warmup * DAY)
     state_w = g.get_state()
@@ -235,6 +236,8 @@ def main(argv=None):
         ms, nl = g.last_timing()
         kernel_ms += ms
         launches += nl
+        if (d + 1) % 30 == 0:
+            print(f"bench: {d + 1} of {args.steps} days", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     d1 = g.diagnostics()
@@ -250,6 +253,9 @@ def main(argv=None):
     xs = g.get_state()
     land_cells = np.repeat(dom.mask, K) != 0
     min_state = float(xs[:, ~land_cells].min())
+    # C/N/P closure over the whole run (warm-up included), outside the timing
+    res = budget.closure(inv_start, g.inventory(), g.budget())
+    closure_max = {e: float(r[wet].max()) for e, r in res.items()}
     g.close()
     default_workload = args.n == 300 and args.seed == 0 and args.arith == 1 and args.emp_weights == 0
     traffic = DEFAULT_TRAFFIC_BYTES if default_workload else None
@@ -297,6 +303,7 @@ def main(argv=None):
         "rejected_attempts": int(rej.sum()),
         "newton_iterations": int(newt.sum()),
         "min_state": min_state,
+        "budget_closure_max": closure_max,
         "input_generation_s": gen_s,
     }
     if not args.no_cpu:
