@@ -24,7 +24,7 @@ FC = os.environ.get("SEDCOL_FC", "/opt/rocm/bin/amdflang")
 ARCH = os.environ.get("SEDCOL_ARCH", "gfx950")
 
 HIP_FLAGS = [
-    "-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-shared",
+    "-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC",
     # FP64 results must follow the written operation order (fused
     # multiply-adds are explicit fma() calls)
     "-ffp-contract=off",
@@ -38,22 +38,48 @@ def _run(cmd, cwd=None):
     subprocess.run([str(c) for c in cmd], check=True, cwd=cwd)
 
 
-def build_library(out: Path | None = None, extra=()) -> Path:
-    out = Path(out) if out else PKG / "libsedcol.so"
-    srcs = [CSRC / "sedcol_kernels.hip", CSRC / "sedcol_api.hip"]
+LIB_SOURCES = [CSRC / "sedcol_kernels.hip", CSRC / "sedcol_api.hip"]
+SURR_SOURCES = [CSRC / "sedsurr.hip"]
+OBJ_DIR = PKG / "_build"
+
+
+def _compile(srcs, extra=()) -> list[Path]:
+    """Compile HIP sources to objects, all at once in separate processes."""
+    OBJ_DIR.mkdir(exist_ok=True)
+    jobs = []
+    for src in srcs:
+        obj = OBJ_DIR / (Path(src).stem + ".o")
+        cmd = [HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-I{CSRC}", "-c", src, "-o", obj]
+        print("+", " ".join(str(c) for c in cmd), flush=True)
+        jobs.append((subprocess.Popen([str(c) for c in cmd]), cmd, obj))
+    failed = [cmd for proc, cmd, _ in jobs if proc.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, [str(c) for c in failed[0]])
+    return [obj for _, _, obj in jobs]
+
+
+def _link(objs, out: Path) -> Path:
     tmp = out.with_suffix(".so.tmp")
-    _run([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-I{CSRC}", *srcs, "-o", tmp])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
+
+
+def build_library(out: Path | None = None, extra=()) -> Path:
+    return _link(_compile(LIB_SOURCES, extra), Path(out) if out else PKG / "libsedcol.so")
 
 
 def build_surrogate(out: Path | None = None, extra=()) -> Path:
     """libsedsurr.so: the spin-up surrogate's training and inference kernels."""
-    out = Path(out) if out else PKG / "libsedsurr.so"
-    tmp = out.with_suffix(".so.tmp")
-    _run([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-I{CSRC}", CSRC / "sedsurr.hip", "-o", tmp])
-    os.replace(tmp, out)
-    return out
+    return _link(_compile(SURR_SOURCES, extra), Path(out) if out else PKG / "libsedsurr.so")
+
+
+def build_all(extra=()) -> tuple[Path, Path]:
+    """Both libraries, with every source compiled in parallel."""
+    objs = _compile(LIB_SOURCES + SURR_SOURCES, extra)
+    lib = _link(objs[: len(LIB_SOURCES)], PKG / "libsedcol.so")
+    surr = _link(objs[len(LIB_SOURCES):], PKG / "libsedsurr.so")
+    return lib, surr
 
 
 def build_fortran(lib: Path | None = None) -> Path | None:
@@ -73,8 +99,7 @@ def build_fortran(lib: Path | None = None) -> Path | None:
 
 
 def main(argv=None):
-    build_library()
-    build_surrogate()
+    build_all()
     if (FSRC / "sedcol_hip.F90").exists():
         build_fortran()
 
