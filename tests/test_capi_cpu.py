@@ -67,3 +67,18 @@ def test_descriptor_errors_need_no_device():
             assert e.code == capi.SEDCOL_EINVAL, (layout, e)
         else:
             raise AssertionError(f"layout {layout} with nlayers {nl.tolist()} accepted")
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asking for more exits with status 2 and
+    says so, before touching any device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    repo = Path(__file__).resolve().parent.parent
+    r = subprocess.run([sys.executable, str(repo / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
