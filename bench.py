@@ -79,6 +79,71 @@ def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads):
     }
 
 
+def rk4_bench(args):
+    """The plain RK4 case: 4 x 4 columns, K = 30, dt = 360 s, 30 simulated
+    days (7200 steps) in one persistent launch, reference operation order.
+    Beside it the reference's own ode_solver (solver_library.F90 compiled
+    unmodified into oracle/_ref) with the oracle's right-hand side, on one
+    core, from the same state; the final states are compared bitwise."""
+    import torch
+
+    from sedcol import cases, capi
+    from sedcol.params import RUNGE_KUTTA_4
+
+    torch.cuda.init()
+    dom = cases.plain_columns(seed=args.seed)
+    dom.params.arith = 0
+    dom.params.clamp = 0  # ode_solver alone; the caller's clamp is not part of it
+    nsteps = 7200
+    g = capi.SedimentColumns.from_domain(dom)
+    for _ in range(max(args.warmup, 1)):  # untimed: library and kernel start-up
+        g.set_state(dom.conc0)
+        g.run(360.0, nsteps, RUNGE_KUTTA_4)
+    g.set_state(dom.conc0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.run(360.0, nsteps, RUNGE_KUTTA_4)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms, _ = g.last_timing()
+    xg = g.get_state()
+    g.close()
+    colsteps = dom.ncol * nsteps
+    out = {
+        "metric": "FP64 RK4 column-steps/s on 1 MI355X (4x4 columns, K=30, dt 360 s, 30 days)",
+        "value": colsteps / wall,
+        "unit": "column-steps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain RK4 4x4x30, dt 360 s, 7200 steps in one launch, no clamp", "seed": args.seed,
+                   "arith": "reference"},
+        "wall_s": wall,
+        "kernel_ms": ms,
+    }
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle as O
+
+    if O.ref_available():
+        x = np.ascontiguousarray(dom.conc0.copy())
+        orc = O.Oracle(dom, dom.params.si())
+        t0 = time.perf_counter()
+        done = 0
+        while done < nsteps:  # in chunks of 1 simulated day
+            assert orc.ref_ode_solver(x, 360.0, RUNGE_KUTTA_4, nsteps=240) == 0
+            done += 240
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": dom.ncol * nsteps / el, "unit": "column-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"the whole case, {dom.ncol} columns x {nsteps} steps in {el:.2f} s: solver_library.F90 "
+                      "compiled from the reference (oracle/_ref), with the oracle's right-hand side",
+        }
+        out["results_bitwise_equal"] = bool(np.array_equal(xg, x))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def ragged_bench(args):
     """Ragged batch: 1e5 columns, K in 10..60, depth 0.1-0.5 m, one forcing
     record.  After an untimed warm-up the same span is integrated from the
@@ -190,8 +255,9 @@ def main(argv=None):
     ap.add_argument("--arith", type=int, default=1, help="0 reference operation order, 1 fast form")
     ap.add_argument("--emp-weights", type=int, default=0,
                     help="Patankar factor of adaptive EMP: 0 product (default), 1 geometric mean")
-    ap.add_argument("--workload", choices=("shelf", "ragged"), default="shelf",
-                    help="shelf: the benchmark line; ragged: schedule/autotuner comparison on ragged columns")
+    ap.add_argument("--workload", choices=("shelf", "ragged", "rk4"), default="shelf",
+                    help="shelf: the benchmark line; ragged: schedule/autotuner comparison on ragged columns; "
+                         "rk4: the plain 4x4 RK4 case beside the reference's own stepper on the CPU")
     ap.add_argument("--ncol", type=int, default=100000, help="ragged: columns")
     ap.add_argument("--span-hours", type=float, default=24.0, help="ragged: timed span per schedule")
     ap.add_argument("--window", type=int, default=100, help="ragged: attempts per column per window")
@@ -206,6 +272,8 @@ def main(argv=None):
         return 2
     if args.workload == "ragged":
         return ragged_bench(args)
+    if args.workload == "rk4":
+        return rk4_bench(args)
 
     import torch
 
