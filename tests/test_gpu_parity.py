@@ -528,3 +528,25 @@ def test_shelf_sea_full_size_properties(gpu_ready):
     for e, r in res.items():
         print(f"budget {e}: max rel closure {r[wet].max():.3e}", flush=True)
         assert r[wet].max() <= 1e-12
+
+
+def test_ragged_full_size_properties(gpu_ready):
+    """The full ragged batch (1e5 columns, K U{10..60}, 3.5 M cells), first
+    simulated hour from the cold state under adaptive EMP: every
+    concentration finite and non-negative, every column accepted steps, and
+    the C/N/P budget of every column closed to 1e-12 relative."""
+    dom = cases.ragged_columns(ncol=100_000, seed=0)
+    dom.params.arith = 1
+    with capi.SedimentColumns.from_domain(dom) as g:
+        inv0 = g.inventory()
+        g.integrate(0.0, 3600.0)
+        x = g.get_state()
+        b = g.budget()
+        inv1 = g.inventory()
+        d = g.diagnostics()
+    assert np.all(np.isfinite(x)) and np.all(x >= 0.0)
+    assert np.all(d["status"] == 0) and np.all(d["accepted"] > 0)
+    res = budget_residuals(inv0, inv1, b)
+    for e, r in res.items():
+        print(f"ragged budget {e}: max rel closure {r.max():.3e}", flush=True)
+        assert r.max() <= 1e-12
