@@ -504,7 +504,8 @@ def test_rk4_one_year(gpu_ready, arith):
         assert r.max() <= 1e-12
 
 
-def test_shelf_sea_full_size_properties(gpu_ready):
+@pytest.mark.parametrize("weights", [pytest.param(0, id="product"), pytest.param(1, id="geometric")])
+def test_shelf_sea_full_size_properties(gpu_ready, weights):
     """The full 300 x 300 shelf-sea domain (53 211 wet columns), first
     simulated day from the cold state under adaptive EMP: every
     concentration finite and non-negative, land untouched, and the C/N/P
@@ -512,6 +513,7 @@ def test_shelf_sea_full_size_properties(gpu_ready):
     integrated top fluxes - DIC/N2 release + clamp mass)."""
     dom = cases.shelf_sea(n=300, days=2, seed=0)
     dom.params.arith = 1
+    dom.params.emp_weights = weights
     land = np.repeat(dom.mask, dom.nlayers) != 0
     with capi.SedimentColumns.from_domain(dom) as g:
         inv0 = g.inventory()
@@ -530,13 +532,15 @@ def test_shelf_sea_full_size_properties(gpu_ready):
         assert r[wet].max() <= 1e-12
 
 
-def test_ragged_full_size_properties(gpu_ready):
+@pytest.mark.parametrize("weights", [pytest.param(0, id="product"), pytest.param(1, id="geometric")])
+def test_ragged_full_size_properties(gpu_ready, weights):
     """The full ragged batch (1e5 columns, K U{10..60}, 3.5 M cells), first
     simulated hour from the cold state under adaptive EMP: every
     concentration finite and non-negative, every column accepted steps, and
     the C/N/P budget of every column closed to 1e-12 relative."""
     dom = cases.ragged_columns(ncol=100_000, seed=0)
     dom.params.arith = 1
+    dom.params.emp_weights = weights
     with capi.SedimentColumns.from_domain(dom) as g:
         inv0 = g.inventory()
         g.integrate(0.0, 3600.0)
