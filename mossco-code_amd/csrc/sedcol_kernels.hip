@@ -10,6 +10,8 @@
 // Top fluxes are produced on lane 0 (one value per variable and stage); they
 // are staged in LDS and accumulated by lanes 0..NV-1, lane n owning
 // variable n, so no lane carries per-variable flux registers.
+#include <atomic>
+
 #include "sedcol_kernels.hpp"
 
 namespace sc {
@@ -894,20 +896,27 @@ void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *
 #undef SC_FIX
 }
 
-// blocks of a work-queue kernel that are resident at once on this device
+// blocks of a work-queue kernel that are resident at once on the current
+// device, cached per device (threads that race on a first query store the
+// same value)
 template <int LPC, int AR, int BLK, int PW>
 static int emp_resident()
 {
-  static int cached = 0;
-  if (cached == 0) {
-    int dev = 0, per_cu = 0;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive<LPC, AR, BLK, PW>, BLK, 0) != hipSuccess)
-      per_cu = 1;
-    cached = (per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount;
-  }
-  return cached;
+  constexpr int kMaxDev = 64;
+  static std::atomic<int> cached[kMaxDev];  // static storage: starts at zero
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  std::atomic<int> *slot = (dev >= 0 && dev < kMaxDev) ? &cached[dev] : nullptr;
+  int v = slot ? slot->load(std::memory_order_relaxed) : 0;
+  if (v > 0) return v;
+  hipDeviceProp_t prop;
+  int per_cu = 0;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive<LPC, AR, BLK, PW>, BLK, 0) != hipSuccess)
+    per_cu = 1;
+  v = (per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount;
+  if (slot) slot->store(v, std::memory_order_relaxed);
+  return v;
 }
 
 #define SC_DISPATCH_BLK(blk, ...)                                \
