@@ -172,7 +172,8 @@ typedef struct {
 
 /* Replaces type_sed allocation in initialize (fabm_sediment_driver.F90:191-368)
  * and the conc allocation of its callers (fabm_sediment_component.F90:521,
- * examples/standalone/omexdia_p/main.F90:75-80).  Device 0 of the process. */
+ * examples/standalone/omexdia_p/main.F90:75-80).  The handle belongs to the
+ * device current at this call (hipSetDevice beforehand selects it). */
 int sedcol_create(const sedcol_desc *desc, sedcol_handle **out);
 int sedcol_destroy(sedcol_handle *h);
 const char *sedcol_last_error(void);
@@ -223,7 +224,9 @@ int sedcol_get_state(sedcol_handle *h, double *conc);
 
 /* get_rhs (fabm_sediment_driver.F90:575-717): rhs of conc under the current
  * forcing; flux_top_out (nvar, ncol, may be NULL) = intFlux(:,:,1) as stored
- * into fluxes(:,:,n) at :692.  Masked columns return rhs = 0. */
+ * into fluxes(:,:,n) at :692.  Masked columns return rhs = 0.  With a
+ * forcing series (nrec > 1) active there is no single current forcing:
+ * SEDCOL_EINVAL. */
 int sedcol_rhs(sedcol_handle *h, const double *conc, double *rhs, double *flux_top_out);
 
 /* ode_solver(rhs_driver, dt, method) (solver_library.F90:80-189) on the
@@ -276,7 +279,8 @@ int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t rep
  * t_conv (ncol, optional): time after t_start at which the column converged,
  * -1 if it did not (and for land columns).  resid (ncol, optional): final
  * residual.  *n_unconverged (optional): wet columns still above tol.
- * No reference counterpart for the per-column form. */
+ * Runs under one forcing record: SEDCOL_EINVAL while a series (nrec > 1)
+ * is active.  No reference counterpart for the per-column form. */
 int sedcol_spinup(sedcol_handle *h, double t_start, double chunk, double t_max, double tol,
                   const double *cfloor, const double *flux_ref, double flux_rtol,
                   const double *flux_floor, double *t_conv, double *resid,
@@ -348,7 +352,8 @@ int sedcol_reset_budget(sedcol_handle *h);
 int sedcol_get_inventory(sedcol_handle *h, double *inventory);
 int sedcol_get_diagnostics(sedcol_handle *h, sedcol_coldiag *diag); /* ncol entries */
 int sedcol_reset_diagnostics(sedcol_handle *h);
-/* steady-state residual per column: max |rhs| / max(|c|, cfloor[n]) (s-1) */
+/* steady-state residual per column: max |rhs| / max(|c|, cfloor[n]) (s-1);
+ * NaN when any term is NaN.  SEDCOL_EINVAL while a forcing series is active. */
 int sedcol_residual(sedcol_handle *h, const double *cfloor, double *resid);
 /* last_min_dt of the reference ADAPTIVE_EULER (solver_library.F90:42,130-135) */
 int sedcol_get_last_min_dt(sedcol_handle *h, double *last_min_dt);

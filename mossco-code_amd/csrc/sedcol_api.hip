@@ -849,6 +849,7 @@ int sedcol_rhs(sedcol_handle *h, const double *conc, double *rhs, double *flux_t
   if (rc) return rc;
   const DeviceScope dscope(h->device);
   if (!conc || !rhs) return fail(SEDCOL_EINVAL, "null argument");
+  if (h->nrec > 1) return fail(SEDCOL_EINVAL, "a forcing series is active; get_rhs uses one record (sedcol_set_forcing)");
   if ((rc = ensure_scratch(h))) return rc;
   std::vector<double> buf;
   pack_state(h, conc, buf);
@@ -1185,6 +1186,8 @@ int sedcol_spinup(sedcol_handle *h, double t_start, double chunk, double t_max, 
   if (rc) return rc;
   const DeviceScope dscope(h->device);
   if (!cfloor) return fail(SEDCOL_EINVAL, "null cfloor");
+  if (h->nrec > 1)
+    return fail(SEDCOL_EINVAL, "a forcing series is active; spin-up runs under one record (sedcol_set_forcing)");
   if (!std::isfinite(t_start) || !(chunk > 0.0) || !std::isfinite(chunk) || !(t_max >= 0.0) ||
       !std::isfinite(t_max) || !(tol > 0.0))
     return fail(SEDCOL_EINVAL, "need finite t_start, chunk > 0, t_max >= 0, tol > 0");
@@ -1373,6 +1376,7 @@ int sedcol_residual(sedcol_handle *h, const double *cfloor, double *resid)
   if (rc) return rc;
   const DeviceScope dscope(h->device);
   if (!cfloor || !resid) return fail(SEDCOL_EINVAL, "null argument");
+  if (h->nrec > 1) return fail(SEDCOL_EINVAL, "a forcing series is active; the residual uses one record (sedcol_set_forcing)");
   return residual_flux(h, cfloor, resid, nullptr);
 }
 

@@ -275,13 +275,14 @@ __device__ __forceinline__ double lg_sum(double v)
   if constexpr (S >= LPC) return v;
   else return lg_sum<LPC, 2 * S>(v + bfly<S>(v));
 }
+// maximum over the group; a NaN in any lane gives NaN
 template <int LPC, int S = 1>
 __device__ __forceinline__ double lg_max(double v)
 {
   if constexpr (S >= LPC) return v;
   else {
     const double o = bfly<S>(v);
-    return lg_max<LPC, 2 * S>(o > v ? o : v);
+    return lg_max<LPC, 2 * S>((o > v || o != o) ? o : v);
   }
 }
 // (P, D) product-with-derivative pairs: (P1,D1)(P2,D2) = (P1 P2, D1 P2 + P1 D2)

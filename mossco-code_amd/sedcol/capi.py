@@ -408,12 +408,14 @@ class SedimentColumns:
 
     # -- convenience ---------------------------------------------------
     @classmethod
-    def from_domain(cls, dom, record=0):
-        """Create, configure and load a cases.Domain (packed layout)."""
+    def from_domain(cls, dom, record=0, series=None):
+        """Create, configure and load a cases.Domain (packed layout).  With
+        more than one forcing record the whole series is set, unless series
+        is False: then record `record` is the current forcing."""
         s = cls(dom.nlayers, dom.mask, layout="packed")
         s.set_params(dom.params)
         s.set_geometry(dom.dz, dom.dzc, dom.phi, dom.phiI, dom.bfac, dom.biot)
-        if dom.nrec > 1:
+        if dom.nrec > 1 and series is not False:
             s.set_forcing_series(dom.temp, dom.cbw, dom.fup, dom.t0, dom.dt_rec)
         else:
             s.set_forcing(*dom.record(record))

@@ -85,6 +85,8 @@ def lib():
         L.oc_rsqrt.restype = ctypes.c_double
         L.oc_rsqrt.argtypes = [ctypes.c_double]
         L.oc_kinetics.argtypes = [vp, ctypes.c_double, _pdt, _pdt, _pdt, _pdt]
+        L.oc_set_plant.restype = ctypes.c_int
+        L.oc_set_plant.argtypes = [ctypes.c_int]
         L.oc_rhs.argtypes = [vp, vp, _pdt, _pdt, _pdt]
         L.oc_ode_solver.argtypes = [vp, vp, _pdt, ctypes.c_double, ctypes.c_int, _pdt, _pdt, _pdt, vp, _pdt]
         L.oc_run.argtypes = [vp, vp, _pdt, ctypes.c_double, ctypes.c_int, ctypes.c_int, _pdt, _pdt, _pdt, vp,
@@ -265,6 +267,22 @@ def kinetics(params: dict, fT: float, c):
     sN = np.zeros(1)
     lib().oc_kinetics(ctypes.byref(o), float(fT), _p(x), _p(R), _p(sC), _p(sN))
     return R, sC[0], sN[0]
+
+
+class planted:
+    """Context manager selecting a planted mistake in the oracle's kinetics
+    (tests of the kinetics pins only; 0 = the definition)."""
+
+    def __init__(self, which: int):
+        self.which = int(which)
+
+    def __enter__(self):
+        self.prev = lib().oc_set_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oc_set_plant(self.prev)
+        return False
 
 
 def oc_exp(x: float) -> float:

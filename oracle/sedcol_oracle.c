@@ -132,10 +132,32 @@ double oc_rsqrt(double x)
 
 static inline double nn(double c) { return c < 0.0 ? 0.0 : c; }
 
+/* Planted mistakes, for the tests that show the kinetics pins catch them:
+ * oc_plant = 0 (the default) is the definition; each other value replaces
+ * one term by a plausible wrong variant (see oc_set_plant).  Nothing but
+ * those tests sets it. */
+static int oc_plant = 0;
+
+/* 0 definition; 1 nitrification O2 term as inhibition ks/(O2+ks); 2 hard
+ * production cap min(1, CprodMax/(dL+dS)); 3 one O2 per nitrified N; 4 one
+ * NO3 (and one N2-N) per denitrified C; 5 anoxic O2 inhibition with
+ * kinO2denit; 6 detritus N decaying without the production cap; 7 P release
+ * term PAdsODU/(ODU+PAdsODU); 8 temperature factor referenced to 20 degC;
+ * 9 oxic mineralisation half-saturation kinO2anox; 10 ODU oxidation
+ * half-saturation ksO2oxic; 11 labile detritus P decaying at the semi-labile
+ * rate.  Returns the previous value. */
+int oc_set_plant(int which)
+{
+  const int prev = oc_plant;
+  oc_plant = which;
+  return prev;
+}
+
 /* temperature factor, fabm_sediment_driver.F90:648 (the _ONE_* factor is an
  * exact multiplication by one) */
 static inline double oc_fT(double T)
 {
+  if (oc_plant == 8) return oc_exp(-4500.0 * (1.0 / (T + 273.0) - (1.0 / 293.0)));
   return oc_exp(-4500.0 * (1.0 / (T + 273.0) - (1.0 / 288.0)));
 }
 
@@ -167,12 +189,18 @@ void oc_kinetics(const oc_params *p, double fT, const double *c, double *R,
   double dL = kL * ldC;
   double dS = kS * sdC;
   double lim = p->CprodMax / (p->CprodMax + (dL + dS));
+  if (oc_plant == 2) lim = (dL + dS) > p->CprodMax ? p->CprodMax / (dL + dS) : 1.0;
   double CpL = dL * lim;
   double CpS = dS * lim;
   double Cprod = CpL + CpS;
   double NpL = (kL * ldN) * lim;
   double NpS = (kS * sdN) * lim;
   double Pprod = (kL * dP) * lim;
+  if (oc_plant == 6) {
+    NpL = kL * ldN;
+    NpS = kS * sdN;
+  }
+  if (oc_plant == 11) Pprod = (kS * dP) * lim;
 
   double oxlim, dnlim, anlim, nitlim, o2odu, odulim;
   if (p->arith == 0) {
@@ -182,6 +210,11 @@ void oc_kinetics(const oc_params *p, double fT, const double *c, double *R,
     nitlim = O2 / (O2 + p->ksO2nitri);
     o2odu = O2 / (O2 + p->ksO2oduox);
     odulim = ODU / (ODU + p->PAdsODU);
+    if (oc_plant == 1) nitlim = p->ksO2nitri / (O2 + p->ksO2nitri);
+    if (oc_plant == 5) anlim = (p->kinO2denit / (O2 + p->kinO2denit)) * (p->kinNO3anox / (NO3 + p->kinNO3anox));
+    if (oc_plant == 7) odulim = p->PAdsODU / (ODU + p->PAdsODU);
+    if (oc_plant == 9) oxlim = O2 / (O2 + p->kinO2anox);
+    if (oc_plant == 10) o2odu = O2 / (O2 + p->ksO2oxic);
   } else {
     /* fast form: the six Monod denominators share one reciprocal, taken of
      * their product and split back with prefix/suffix products (valid while
@@ -227,6 +260,11 @@ void oc_kinetics(const oc_params *p, double fT, const double *c, double *R,
   R[OC_ADSP] = Pads - Pdes;
   *sinkC = Cprod;
   *sinkN = 0.8 * Denit;
+  if (oc_plant == 3) R[OC_O2] = -((OxicMin + Nitri) + OduOx);
+  if (oc_plant == 4) {
+    R[OC_NO3] = Nitri - Denit;
+    *sinkN = Denit;
+  }
 }
 
 /* ------------------------------------------------------------------------ */

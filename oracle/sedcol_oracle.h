@@ -116,6 +116,8 @@ void oc_zw_geometry(const oc_zw *zw, const double *dz, int K, double *e1, double
 double oc_rsqrt(double x);
 void oc_kinetics(const oc_params *p, double fT, const double *c, double *R,
                  double *sinkC, double *sinkN);
+/* tests only: select a planted mistake in the kinetics (0 = none) */
+int oc_set_plant(int which);
 int oc_rhs(const oc_params *p, const oc_domain *d, const double *conc,
            double *rhs, double *flux_out);
 int oc_ode_solver(const oc_params *p, const oc_domain *d, double *conc,

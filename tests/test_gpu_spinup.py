@@ -100,3 +100,45 @@ def test_spinup_arguments(gpu_ready, history):
         tc, rs, nu = g.spinup(0.0, DAY, 0.0, 1e-30, spinup.CFLOOR)
         assert nu == dom.ncol and np.all(tc == -1.0)
         assert np.array_equal(g.get_state(), x0)
+
+
+def test_nan_in_a_deep_layer(gpu_ready):
+    """A NaN below the top layers reaches the column's residual (the layer
+    maximum keeps NaN, as the oracle's scan does) and stops the spin-up with
+    SEDCOL_ENAN instead of retiring the column as converged at t = 0."""
+    rng = np.random.Generator(np.random.Philox(11))
+    n = 6
+    s = spinup.draw_inputs(rng, n)
+    dom = spinup.build(s, np.full(n, 30))
+    x = dom.conc0.copy()
+    x[6, dom.off[2] + 5] = np.nan  # NO3, column 2, layer 5
+    ro = O.Oracle(dom, dom.params.si()).residual(x, spinup.CFLOOR)
+    with capi.SedimentColumns.from_domain(dom) as g:
+        g.set_state(x)
+        rg = g.residual(spinup.CFLOOR)
+        assert np.isnan(ro[2]) and np.isnan(rg[2])
+        ok = np.arange(n) != 2
+        assert np.array_equal(rg[ok], ro[ok])
+        with pytest.raises(capi.SedcolError) as e:
+            g.spinup(0.0, DAY, 2 * DAY, 1e30, spinup.CFLOOR)
+        assert e.value.code == capi.SEDCOL_ENAN
+        assert g.diagnostics()["status"][2] == 1
+
+
+def test_single_record_entry_points_refuse_a_series(gpu_ready):
+    """get_rhs, the residual and the spin-up run under one forcing record;
+    with a forcing series active they return SEDCOL_EINVAL."""
+    from sedcol import cases
+
+    dom = cases.shelf_sea(n=10, days=3, seed=2)
+    assert (dom.mask == 0).any()
+    with capi.SedimentColumns.from_domain(dom) as g:
+        for call in (lambda: g.get_rhs(dom.conc0), lambda: g.residual(spinup.CFLOOR),
+                     lambda: g.spinup(0.0, DAY, DAY, 1e-6, spinup.CFLOOR)):
+            with pytest.raises(capi.SedcolError) as e:
+                call()
+            assert e.value.code == capi.SEDCOL_EINVAL
+        g.integrate(0.0, DAY)  # the series itself integrates
+    with capi.SedimentColumns.from_domain(dom, record=1, series=False) as g:
+        g.get_rhs(dom.conc0)
+        g.residual(spinup.CFLOOR)
