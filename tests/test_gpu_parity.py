@@ -285,6 +285,7 @@ def test_masked_columns_untouched(gpu_ready):
     with capi.SedimentColumns.from_domain(full) as g:
         g.integrate(0.0, 86400.0)
         x = g.get_state()
+    with capi.SedimentColumns.from_domain(full, series=False) as g:
         r, f = g.get_rhs(full.conc0)
     land = np.repeat(full.mask, full.nlayers) != 0
     assert np.all(x[:, land] == 1.0e20)
