@@ -229,6 +229,21 @@ int sedcol_get_state(sedcol_handle *h, double *conc);
  * SEDCOL_EINVAL. */
 int sedcol_rhs(sedcol_handle *h, const double *conc, double *rhs, double *flux_top_out);
 
+/* sedcol_rhs without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device or host memory during the call (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as sedcol_rhs, bitwise; it
+ * calls no hipMalloc/hipFree and no allocating routine.  Its staging lives in
+ * a fixed 64 MiB workspace per device (held from library load until the
+ * process ends), so the domain must fit: (2 NV ncell + NV ncol) doubles in
+ * the packed layout, (3 NV ncell + NV ncol) in the Fortran layout, i.e. up
+ * to about 380 000 (packed) or 250 000 (Fortran) cells; larger domains return
+ * SEDCOL_EINVAL.  Every call on one GPU shares that workspace and it keeps
+ * the last call's contents (each call fills what it reads): the library runs
+ * one such call per device at a time (a lock), so calls from several host
+ * threads are serialised; nothing else may touch the workspace. */
+int sedcol_rhs_noalloc(sedcol_handle *h, const double *conc, double *rhs, double *flux_top_out);
+
 /* ode_solver(rhs_driver, dt, method) (solver_library.F90:80-189) on the
  * caller's conc, updated in place (the drop-in call used at
  * fabm_sediment_component.F90:1715 and main.F90:123).  flux_top_out = top

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -34,6 +35,35 @@ int fail(int code, const std::string &msg)
   } while (0)
 
 constexpr int kLpc[3] = {16, 32, 64};
+
+// Workspace of sedcol_rhs_noalloc: staging of its input state, its
+// right-hand side and top fluxes, and (Fortran layout) of the caller's
+// array, so that the call needs no allocation.  One copy per device, held
+// from module load until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device.
+constexpr size_t kNoallocBytes = size_t(64) << 20;
+__device__ __align__(256) double scratch[kNoallocBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_noalloc_mu[kMaxDevices];  // one call at a time per device
+
+// Device address of `scratch` on device dev (which must be current): looked
+// up once per device, at the first sedcol_create there.
+hipError_t scratch_address(int dev, double **out)
+{
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
 
 }  // namespace
 
@@ -65,6 +95,7 @@ struct sedcol_handle {
   double *d_scr = nullptr;  // NV*ncell scratch (rhs in/out, adaptive Euler rates)
   double *d_scr2 = nullptr; // NV*ncell scratch
   double *d_small = nullptr;  // NV*ncol + ncol scratch
+  double *d_noalloc = nullptr;  // this device's sedcol_rhs_noalloc workspace (not owned)
   int *d_flag = nullptr;     // [0] error flags, [1..3] work-queue counters per bin, [4] windowed columns pending
   int *h_flag = nullptr;     // pinned host copy of d_flag[0..4]
   int *d_work = nullptr;     // ncol: attempts of each column in the last adaptive launch
@@ -518,6 +549,7 @@ int sedcol_create(const sedcol_desc *desc, sedcol_handle **out)
       return cleanup(fail(SEDCOL_EHIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
   } while (0)
   HIPOK_C(hipGetDevice(&h->device));
+  HIPOK_C(scratch_address(h->device, &h->d_noalloc));
   HIPOK_C(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPOK_C(hipEventCreate(&h->ev0));
   HIPOK_C(hipEventCreate(&h->ev1));
@@ -868,6 +900,41 @@ int sedcol_rhs(sedcol_handle *h, const double *conc, double *rhs, double *flux_t
   HIPOK(hipStreamSynchronize(h->stream));
   unpack_state(h, buf, rhs);
   if (flux_top_out) std::memcpy(flux_top_out, fl.data(), sizeof(double) * fl.size());
+  return SEDCOL_OK;
+}
+
+int sedcol_rhs_noalloc(sedcol_handle *h, const double *conc, double *rhs, double *flux_top_out)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  const DeviceScope dscope(h->device);
+  if (!conc || !rhs) return fail(SEDCOL_EINVAL, "null argument");
+  if (h->nrec > 1) return fail(SEDCOL_EINVAL, "a forcing series is active; get_rhs uses one record (sedcol_set_forcing)");
+  const bool fortran = h->layout == SEDCOL_LAYOUT_FORTRAN;
+  const size_t cells = (size_t)NV * (size_t)h->ncell, fl = (size_t)NV * (size_t)h->ncol;
+  const size_t need = sizeof(double) * ((fortran ? 3 : 2) * cells + fl);
+  if (need > kNoallocBytes)
+    return fail(SEDCOL_EINVAL, "sedcol_rhs_noalloc: the domain needs " + std::to_string(need) +
+                                   " bytes of workspace, more than its fixed " + std::to_string(kNoallocBytes) +
+                                   " (use sedcol_rhs)");
+  if (!h->d_noalloc) return fail(SEDCOL_EHIP, "sedcol_rhs_noalloc: no workspace address on this device");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu[h->device]);
+  double *in = h->d_noalloc, *out = in + cells, *flux = out + cells, *raw = flux + fl;
+  HIPOK(hipMemcpyAsync(fortran ? raw : in, conc, sizeof(double) * cells, hipMemcpyHostToDevice, h->stream));
+  if (fortran) sc::launch_layout(h->stream, raw, in, h->ncol, h->kmax, true);
+  // land columns keep rhs = 0 and no top flux; fill kernels, not memsets
+  sc::launch_fill_dbl(h->stream, out, (long long)cells, 0.0);
+  sc::launch_fill_dbl(h->stream, flux, (long long)fl, 0.0);
+  const Dev d = h->dev();
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty())
+      sc::launch_rhs(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, 0, in, out, flux);
+  if (fortran) sc::launch_layout(h->stream, out, raw, h->ncol, h->kmax, false);
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(rhs, fortran ? raw : out, sizeof(double) * cells, hipMemcpyDeviceToHost, h->stream));
+  if (flux_top_out)
+    HIPOK(hipMemcpyAsync(flux_top_out, flux, sizeof(double) * fl, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
   return SEDCOL_OK;
 }
 

@@ -1022,6 +1022,43 @@ void launch_fill_int(hipStream_t st, int *p, int n, int v)
   hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
 }
 
+__global__ void k_fill_dbl(double *p, long long n, double v)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill_dbl(hipStream_t st, double *p, long long n, double v)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_dbl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+// Fortran state conc(i,j,k,n) = f[(n K + k) ncol + c] against the packed
+// state x[n ncell + c K + k] of a uniform grid (ncell = ncol K); one thread
+// per element, in Fortran order (reads or writes of f coalesced).
+__global__ void k_layout(const double *src, double *dst, int ncol, int K, int to_packed)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long plane = (long long)ncol * K;
+  if (i >= (long long)NV * plane) return;
+  const long long n = i / plane;
+  const long long r = i - n * plane;
+  const long long k = r / ncol;
+  const long long c = r - k * ncol;
+  const long long j = n * plane + c * K + k;
+  if (to_packed) dst[j] = src[i];
+  else dst[i] = src[j];
+}
+
+void launch_layout(hipStream_t st, const double *src, double *dst, int ncol, int K, bool to_packed)
+{
+  const long long n = (long long)NV * ncol * K;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_layout, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, ncol, K,
+                     to_packed ? 1 : 0);
+}
+
 void launch_rhs_store(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
                       double *r_out, double *s_out)
 {

@@ -68,6 +68,11 @@ void launch_zw_geometry(hipStream_t st, const Dev &d, const int *list, int n, do
                         double *cum);
 // v into p[0..n-1] (a kernel, so that it can be captured in a graph)
 void launch_fill_int(hipStream_t st, int *p, int n, int v);
+void launch_fill_dbl(hipStream_t st, double *p, long long n, double v);
+// state of a uniform grid between the Fortran layout (ncol, K, NV) and the
+// packed one (NV, ncol*K): src Fortran -> dst packed when to_packed, else
+// src packed -> dst Fortran
+void launch_layout(hipStream_t st, const double *src, double *dst, int ncol, int K, bool to_packed);
 // boundary conditions of the listed columns from the fields at the soil
 // surface into forcing record 0 (present: bit n set when variable n's
 // field exists)

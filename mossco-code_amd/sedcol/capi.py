@@ -34,6 +34,7 @@ NBUD = 16
 EXPORTS = (
     "sedcol_create", "sedcol_destroy", "sedcol_last_error", "sedcol_set_params", "sedcol_set_geometry",
     "sedcol_set_forcing", "sedcol_set_forcing_series", "sedcol_set_state", "sedcol_get_state", "sedcol_rhs",
+    "sedcol_rhs_noalloc",
     "sedcol_step", "sedcol_run", "sedcol_integrate", "sedcol_get_fluxes", "sedcol_get_budget",
     "sedcol_reset_budget", "sedcol_get_inventory", "sedcol_get_diagnostics", "sedcol_reset_diagnostics",
     "sedcol_residual", "sedcol_get_last_min_dt", "sedcol_synchronize", "sedcol_last_timing",
@@ -119,6 +120,7 @@ def lib() -> ctypes.CDLL:
             "sedcol_set_state": (ctypes.c_int, [vp, pd]),
             "sedcol_get_state": (ctypes.c_int, [vp, pd]),
             "sedcol_rhs": (ctypes.c_int, [vp, pd, pd, pd]),
+            "sedcol_rhs_noalloc": (ctypes.c_int, [vp, pd, pd, pd]),
             "sedcol_step": (ctypes.c_int, [vp, dbl, i32, pd, pd]),
             "sedcol_run": (ctypes.c_int, [vp, dbl, i32, i32, i32]),
             "sedcol_integrate": (ctypes.c_int, [vp, dbl, dbl]),
@@ -282,6 +284,14 @@ class SedimentColumns:
         r = np.empty_like(x)
         f = np.empty((NVAR, self.ncol))
         check(lib().sedcol_rhs(self._h, _pd(x), _pd(r), _pd(f)))
+        return r, f
+
+    def get_rhs_noalloc(self, conc):
+        """get_rhs through sedcol_rhs_noalloc (no allocation inside the call)."""
+        x = _c(conc)
+        r = np.empty_like(x)
+        f = np.empty((NVAR, self.ncol))
+        check(lib().sedcol_rhs_noalloc(self._h, _pd(x), _pd(r), _pd(f)))
         return r, f
 
     def ode_solver(self, conc, dt, method):
