@@ -44,7 +44,7 @@ DAY = 86400.0
 DEFAULT_TRAFFIC_BYTES = (420649 * 2.0 + 272322 * 1.0) * 1024
 
 
-def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads):
+def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads, cols_per_thread=32):
     """Oracle (plain C, OpenMP over columns) on a sample of wet columns,
     starting from the same state; stops after `seconds` of work."""
     sys.path.insert(0, str(REPO / "oracle"))
@@ -52,7 +52,7 @@ def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads):
     from sedcol import cases
 
     wet = np.flatnonzero(dom.mask == 0)
-    ncols = min(wet.size, 32 * nthreads)
+    ncols = min(wet.size, cols_per_thread * nthreads)
     cols = wet[np.linspace(0, wet.size - 1, ncols).astype(np.int64)]
     sub = dom.subset(cols)
     off = dom.off
@@ -351,10 +351,12 @@ def main(argv=None):
             "seed": args.seed,
             "arith": "fast" if args.arith else "reference",
             "emp_weights": "geometric" if args.emp_weights else "product",
+            "forcing_phase": f"days {args.warmup}-{args.warmup + args.steps - 1} of a seasonal cycle whose "
+                             "coldest, most oxygenated bottom water is on day 0 (sin(2 pi d/365 - pi/2))",
         },
         "roofline": {
-            "bound": "mfma",
-            "pipe": "FP64 VALU (MI355X FP64 peak 78.6 TFLOP/s; no matrix work on this path)",
+            "bound": "fp64-valu",
+            "pipe": "FP64 VALU (MI355X FP64 vector peak 78.6 TFLOP/s; no matrix work on this path)",
             "kernel": "k_emp_adaptive<32>",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
@@ -377,6 +379,9 @@ def main(argv=None):
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(dom, state_w, args.warmup, args.steps, args.cpu_seconds, nthreads)
+        # the same oracle on one host core, a smaller sample of the same days
+        one = cpu_baseline(dom, state_w, args.warmup, args.steps, args.cpu_seconds / 2, 1, cols_per_thread=24)
+        out["cpu_baseline"]["one_core"] = {k: one[k] for k in ("value", "unit", "cores", "sample")}
     print(json.dumps(out), flush=True)
     return 0
 

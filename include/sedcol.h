@@ -58,7 +58,9 @@ enum sedcol_method {
   SEDCOL_RUNGE_KUTTA_4_38 = 3,
   SEDCOL_EMP1 = 4,           /* extended modified Patankar, 1st order */
   SEDCOL_EMP2 = 5,           /* extended modified Patankar, 2nd order */
-  SEDCOL_EMP_ADAPTIVE = 6    /* EMP-1/EMP-2 pair, per-column step size */
+  SEDCOL_EMP_ADAPTIVE = 6,   /* EMP-1/EMP-2 pair, per-column step size */
+  SEDCOL_ROS2_ADAPTIVE = 7   /* linearly implicit ROS2 W-method, per-column
+                                step size (sedcol_params.adaptive_method) */
 };
 
 enum sedcol_status {
@@ -124,6 +126,12 @@ typedef struct {
                                  (Bruggeman et al. 2007), 1 their geometric
                                  mean (the same properties; long columns are
                                  not throttled by their component count) */
+  int32_t adaptive_method;    /* stepper of sedcol_integrate / sedcol_spinup:
+                                 0 or SEDCOL_EMP_ADAPTIVE the adaptive EMP pair,
+                                 SEDCOL_ROS2_ADAPTIVE the linearly implicit
+                                 ROS2 W-method (reaction Jacobian per cell times
+                                 the per-species transport factor, one
+                                 correction; positivity by step rejection) */
 } sedcol_params;
 
 /* Per-column diagnostics (sedcol_get_diagnostics), the per-column form of

@@ -22,7 +22,7 @@ module sedcol_hip
   integer(c_int32_t), parameter, public :: SEDCOL_NVAR = 11
   integer(c_int32_t), parameter, public :: SEDCOL_EULER = 0, SEDCOL_RUNGE_KUTTA_4 = 1, &
     SEDCOL_ADAPTIVE_EULER = 2, SEDCOL_RUNGE_KUTTA_4_38 = 3, SEDCOL_EMP1 = 4, SEDCOL_EMP2 = 5, &
-    SEDCOL_EMP_ADAPTIVE = 6
+    SEDCOL_EMP_ADAPTIVE = 6, SEDCOL_ROS2_ADAPTIVE = 7
   integer(c_int32_t), parameter, public :: SEDCOL_LAYOUT_FORTRAN = 0, SEDCOL_LAYOUT_PACKED = 1
   integer(c_int32_t), parameter, public :: SEDCOL_ARITH_REFERENCE = 0, SEDCOL_ARITH_FAST = 1
 
@@ -41,7 +41,7 @@ module sedcol_hip
     real(c_double) :: rPads = (0.01d0*20.d0)/86400.d0, PAdsODU = 70.d0
     real(c_double) :: emp_rtol = 1.d-6, emp_atol = 1.d-6, emp_dt_min = 1.d0, emp_dt_max = 3600.d0
     integer(c_int32_t) :: bcup_dis = 2, bcup_part = 1, bcdown = 3, clamp = 1
-    integer(c_int32_t) :: arith = 0, emp_weights = 0
+    integer(c_int32_t) :: arith = 0, emp_weights = 0, adaptive_method = 0
   end type
 
   interface

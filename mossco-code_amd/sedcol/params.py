@@ -33,7 +33,7 @@ ELEMENT_VARS = {
 
 # method ids (solver_library.F90:32-35, plus the EMP methods)
 EULER, RUNGE_KUTTA_4, ADAPTIVE_EULER, RUNGE_KUTTA_4_38 = 0, 1, 2, 3
-EMP1, EMP2, EMP_ADAPTIVE = 4, 5, 6
+EMP1, EMP2, EMP_ADAPTIVE, ROS2_ADAPTIVE = 4, 5, 6, 7
 
 
 @dataclass
@@ -72,6 +72,7 @@ class SedParams:
     clamp: int = 1
     arith: int = 0                     # 0 reference operation order, 1 fast form
     emp_weights: int = 0               # Patankar factor: 0 product, 1 geometric mean
+    adaptive_method: int = 0           # stepper of integrate/spinup: 0 or 6 EMP, 7 ROS2
 
     def si(self) -> dict:
         """Flat dict in the field order of sedcol_params / oc_params."""
@@ -107,6 +108,7 @@ class SedParams:
             "clamp": int(self.clamp),
             "arith": int(self.arith),
             "emp_weights": int(self.emp_weights),
+            "adaptive_method": int(self.adaptive_method),
         }
 
 
@@ -121,7 +123,7 @@ PARAM_FIELDS = [
     ("emp_dt_min", ctypes.c_double), ("emp_dt_max", ctypes.c_double),
     ("bcup_dis", ctypes.c_int32), ("bcup_part", ctypes.c_int32),
     ("bcdown", ctypes.c_int32), ("clamp", ctypes.c_int32),
-    ("arith", ctypes.c_int32), ("emp_weights", ctypes.c_int32),
+    ("arith", ctypes.c_int32), ("emp_weights", ctypes.c_int32), ("adaptive_method", ctypes.c_int32),
 ]
 
 
@@ -139,5 +141,5 @@ def params_dict(p: SedParams | dict) -> dict:
 __all__ = [
     "VARIABLES", "NVAR", "PARTICULATE", "ELEMENT_VARS", "SedParams", "OmexdiaNamelist",
     "PARAM_FIELDS", "fill_struct", "params_dict", "asdict",
-    "EULER", "RUNGE_KUTTA_4", "ADAPTIVE_EULER", "RUNGE_KUTTA_4_38", "EMP1", "EMP2", "EMP_ADAPTIVE",
+    "EULER", "RUNGE_KUTTA_4", "ADAPTIVE_EULER", "RUNGE_KUTTA_4_38", "EMP1", "EMP2", "EMP_ADAPTIVE", "ROS2_ADAPTIVE",
 ]

@@ -29,7 +29,7 @@ _PARAM_FIELDS = [
     ("emp_dt_min", ctypes.c_double), ("emp_dt_max", ctypes.c_double),
     ("bcup_dis", ctypes.c_int32), ("bcup_part", ctypes.c_int32),
     ("bcdown", ctypes.c_int32), ("clamp", ctypes.c_int32),
-    ("arith", ctypes.c_int32), ("emp_weights", ctypes.c_int32),
+    ("arith", ctypes.c_int32), ("emp_weights", ctypes.c_int32), ("adaptive_method", ctypes.c_int32),
 ]
 
 
@@ -85,6 +85,8 @@ def lib():
         L.oc_rsqrt.restype = ctypes.c_double
         L.oc_rsqrt.argtypes = [ctypes.c_double]
         L.oc_kinetics.argtypes = [vp, ctypes.c_double, _pdt, _pdt, _pdt, _pdt]
+        L.oc_kin_jac_ops.restype = ctypes.c_int
+        L.oc_kin_jac_ops.argtypes = [vp, ctypes.c_double, _pdt, ctypes.c_double, _pdt, _pdt, _pdt, _pdt]
         L.oc_set_plant.restype = ctypes.c_int
         L.oc_set_plant.argtypes = [ctypes.c_int]
         L.oc_rhs.argtypes = [vp, vp, _pdt, _pdt, _pdt]
@@ -267,6 +269,19 @@ def kinetics(params: dict, fT: float, c):
     sN = np.zeros(1)
     lib().oc_kinetics(ctypes.byref(o), float(fT), _p(x), _p(R), _p(sC), _p(sN))
     return R, sC[0], sN[0]
+
+
+def kin_jac_ops(params: dict, fT: float, c, g: float, z):
+    """(J_R z, (I - g J_R)^-1 z or None if the solve fails, grad Cprod . z,
+    grad 0.8 Denit . z) of the local-rate Jacobian used by method 7."""
+    o = OcParams()
+    for name, _ in _PARAM_FIELDS:
+        setattr(o, name, params[name])
+    x = np.ascontiguousarray(c, dtype=np.float64)
+    zz = np.ascontiguousarray(z, dtype=np.float64)
+    Jz, w, gs = np.zeros(NV), np.zeros(NV), np.zeros(2)
+    rc = lib().oc_kin_jac_ops(ctypes.byref(o), float(fT), _p(x), float(g), _p(zz), _p(Jz), _p(w), _p(gs))
+    return Jz, (None if rc else w), gs[0], gs[1]
 
 
 class planted:

@@ -933,6 +933,444 @@ static int col_emp_adaptive(const oc_params *p, const colview *v, double *c,
 }
 
 /* ------------------------------------------------------------------------ */
+/* linearly implicit Rosenbrock stepper, method 7 (project definition)       */
+/* ------------------------------------------------------------------------ */
+
+/* ROS2 (Verwer, Spee, Blom and Hundsdorfer 1999) as a W-method:
+ *   W k1 = f(c),  W k2 = f(c + h k1) - 2 k1,  c+ = c + h (3/2 k1 + 1/2 k2),
+ * gamma = 1 + 1/sqrt(2), with the first-order c + h k1 as the embedded
+ * solution (error estimate h (k1 + k2)/2).  W approximates I - gamma h J by
+ * the product of the per-species tridiagonal transport factor
+ * B = I - gamma h J_T and the per-cell reaction factor A = I - gamma h J_R,
+ * plus one correction with the full I - gamma h (J_T + J_R):
+ *   k0 = A^-1 B^-1 r,  k = k0 + A^-1 B^-1 (r - (I - gamma h (J_T + J_R)) k0).
+ * J_R is the analytic Jacobian of the local rates (oc_kinetics, reference
+ * form, at the clamped state), J_T that of the column transport (with the
+ * state dependence of a Dirichlet top flux).  A conserves P, B every
+ * variable's column inventory up to its Dirichlet top flux, so the top
+ * fluxes and the C and N sinks of a step follow exactly from the stage
+ * rates, the linearised top flux and the gradients of Cprod and 0.8 Denit
+ * (see col_ros2_adaptive).  Positivity: a step whose new state has a
+ * component below -1e-3 atol is rejected; smaller negative values are set
+ * to 0 and their mass booked as clamp mass. */
+#define ROS_GAMMA 1.7071067811865475
+
+typedef struct {
+  double cl0, cl1, cs0, cs1;      /* dCpL/d(ldC,sdC), dCpS/d(ldC,sdC) */
+  double nl0, nl1, nl2;           /* dNpL/d(ldC,sdC,ldN) */
+  double ns0, ns1, ns3;           /* dNpS/d(ldC,sdC,sdN) */
+  double pp0, pp1, pp4;           /* dPprod/d(ldC,sdC,detP) */
+  double ox0, ox1, ox5, ox6;      /* dOxicMin/d(ldC,sdC,O2,NO3) */
+  double dn0, dn1, dn5, dn6;      /* dDenit/d(...) */
+  double an0, an1, an5, an6;      /* dAnoxMin/d(...) */
+  double ni5, ni7;                /* dNitri/d(O2,NH4) */
+  double od5, od8;                /* dOduOx/d(O2,ODU) */
+  double pa5, pa9;                /* dPads/d(O2,PO4) */
+  double pd8, pd10;               /* dPdes/d(ODU,adsP) */
+  double gc0, gc1;                /* dCprod/d(ldC,sdC) */
+} oc_kjac;
+
+/* Jacobian of the local rates at c (evaluated at max(c, 0)) */
+void oc_kin_jac(const oc_params *p, double fT, const double *c, oc_kjac *J)
+{
+  const double ldC = nn(c[0]), sdC = nn(c[1]), ldN = nn(c[2]), sdN = nn(c[3]);
+  const double dP = nn(c[4]), O2 = nn(c[5]), NO3 = nn(c[6]), NH4 = nn(c[7]);
+  const double ODU = nn(c[8]), PO4 = nn(c[9]), aP = nn(c[10]);
+  const double kL = fT * p->rL, kS = fT * p->rS;
+  const double dL = kL * ldC, dS = kS * sdC;
+  const double den = p->CprodMax + (dL + dS);
+  const double lim = p->CprodMax / den;
+  const double q = lim / den;
+  J->cl0 = kL * (lim - dL * q);
+  J->cl1 = -((dL * kS) * q);
+  J->cs0 = -((dS * kL) * q);
+  J->cs1 = kS * (lim - dS * q);
+  J->gc0 = J->cl0 + J->cs0;
+  J->gc1 = J->cl1 + J->cs1;
+  const double aN = kL * ldN, bN = kS * sdN, aP4 = kL * dP;
+  J->nl0 = -((aN * kL) * q);
+  J->nl1 = -((aN * kS) * q);
+  J->nl2 = kL * lim;
+  J->ns0 = -((bN * kL) * q);
+  J->ns1 = -((bN * kS) * q);
+  J->ns3 = kS * lim;
+  J->pp0 = -((aP4 * kL) * q);
+  J->pp1 = -((aP4 * kS) * q);
+  J->pp4 = kL * lim;
+  const double Cprod = dL * lim + dS * lim;
+  /* partition weights and their derivatives */
+  const double qo = O2 + p->ksO2oxic;
+  const double oxl = O2 / qo;
+  const double oxl5 = p->ksO2oxic / (qo * qo);
+  const double qd = O2 + p->kinO2denit, qn = NO3 + p->ksNO3denit;
+  const double wd = p->kinO2denit / qd, mn = NO3 / qn;
+  const double dnl = wd * mn;
+  const double dnl5 = -(dnl / qd);
+  const double dnl6 = wd * (p->ksNO3denit / (qn * qn));
+  const double qa = O2 + p->kinO2anox, qb = NO3 + p->kinNO3anox;
+  const double anl = (p->kinO2anox / qa) * (p->kinNO3anox / qb);
+  const double anl5 = -(anl / qa);
+  const double anl6 = -(anl / qb);
+  const double rs = 1.0 / ((oxl + dnl) + anl);
+  const double S5 = (oxl5 + dnl5) + anl5, S6 = dnl6 + anl6;
+  const double shox = oxl * rs, shdn = dnl * rs, shan = anl * rs;
+  J->ox0 = J->gc0 * shox;
+  J->ox1 = J->gc1 * shox;
+  J->ox5 = Cprod * ((oxl5 - shox * S5) * rs);
+  J->ox6 = Cprod * ((0.0 - shox * S6) * rs);
+  J->dn0 = J->gc0 * shdn;
+  J->dn1 = J->gc1 * shdn;
+  J->dn5 = Cprod * ((dnl5 - shdn * S5) * rs);
+  J->dn6 = Cprod * ((dnl6 - shdn * S6) * rs);
+  J->an0 = J->gc0 * shan;
+  J->an1 = J->gc1 * shan;
+  J->an5 = Cprod * ((anl5 - shan * S5) * rs);
+  J->an6 = Cprod * ((anl6 - shan * S6) * rs);
+  const double rn = fT * p->rnit, qi = O2 + p->ksO2nitri;
+  J->ni7 = rn * (O2 / qi);
+  J->ni5 = (rn * NH4) * (p->ksO2nitri / (qi * qi));
+  const double ro = fT * p->rODUox, qu = O2 + p->ksO2oduox;
+  const double o2u = O2 / qu, o2u5 = p->ksO2oduox / (qu * qu);
+  J->od8 = ro * o2u;
+  J->od5 = (ro * ODU) * o2u5;
+  const double kP = fT * p->rPads, qp = ODU + p->PAdsODU;
+  J->pa9 = kP * o2u;
+  J->pa5 = (kP * PO4) * o2u5;
+  J->pd10 = kP * (ODU / qp);
+  J->pd8 = (kP * aP) * (p->PAdsODU / (qp * qp));
+}
+
+/* out = J_R z */
+static void kjac_apply(const oc_kjac *J, const double *z, double *o)
+{
+  o[0] = -(J->cl0 * z[0] + J->cl1 * z[1]);
+  o[1] = -(J->cs0 * z[0] + J->cs1 * z[1]);
+  o[2] = -((J->nl0 * z[0] + J->nl1 * z[1]) + J->nl2 * z[2]);
+  o[3] = -((J->ns0 * z[0] + J->ns1 * z[1]) + J->ns3 * z[3]);
+  o[4] = -((J->pp0 * z[0] + J->pp1 * z[1]) + J->pp4 * z[4]);
+  const double ox = (J->ox0 * z[0] + J->ox1 * z[1]) + (J->ox5 * z[5] + J->ox6 * z[6]);
+  const double dn = (J->dn0 * z[0] + J->dn1 * z[1]) + (J->dn5 * z[5] + J->dn6 * z[6]);
+  const double an = (J->an0 * z[0] + J->an1 * z[1]) + (J->an5 * z[5] + J->an6 * z[6]);
+  const double ni = J->ni5 * z[5] + J->ni7 * z[7];
+  const double od = J->od5 * z[5] + J->od8 * z[8];
+  const double pa = J->pa5 * z[5] + J->pa9 * z[9];
+  const double pd = J->pd8 * z[8] + J->pd10 * z[10];
+  o[5] = -((ox + 2.0 * ni) + od);
+  o[6] = ni - 0.8 * dn;
+  o[7] = (0.0 - (o[2] + o[3])) - ni;
+  o[8] = an - od;
+  o[9] = ((0.0 - o[4]) - pa) + pd;
+  o[10] = pa - pd;
+}
+
+/* gradients of the C sink (Cprod) and N sink (0.8 Denit) applied to z */
+static inline double kjac_sinkC(const oc_kjac *J, const double *z) { return J->gc0 * z[0] + J->gc1 * z[1]; }
+static inline double kjac_sinkN(const oc_kjac *J, const double *z)
+{
+  return 0.8 * ((J->dn0 * z[0] + J->dn1 * z[1]) + (J->dn5 * z[5] + J->dn6 * z[6]));
+}
+
+/* (I - g J_R) w = z by the block structure of J_R: detritus C (2x2), then
+ * detritus N and P, then O2/NO3/NH4/ODU (NH4 and ODU eliminated, 2x2 in O2
+ * and NO3), then PO4/adsP (2x2).  Returns 1 if a pivot is not positive. */
+static int kjac_solve(const oc_kjac *J, double g, const double *z, double *w)
+{
+  const double a00 = 1.0 + g * J->cl0, a01 = g * J->cl1, a10 = g * J->cs0, a11 = 1.0 + g * J->cs1;
+  const double det = a00 * a11 - a01 * a10;
+  if (!(det > 0.0)) return 1;
+  const double idet = 1.0 / det;
+  w[0] = (a11 * z[0] - a01 * z[1]) * idet;
+  w[1] = (a00 * z[1] - a10 * z[0]) * idet;
+  w[2] = (z[2] - g * (J->nl0 * w[0] + J->nl1 * w[1])) / (1.0 + g * J->nl2);
+  w[3] = (z[3] - g * (J->ns0 * w[0] + J->ns1 * w[1])) / (1.0 + g * J->ns3);
+  w[4] = (z[4] - g * (J->pp0 * w[0] + J->pp1 * w[1])) / (1.0 + g * J->pp4);
+  const double r5 = z[5] - g * (J->ox0 * w[0] + J->ox1 * w[1]);
+  const double r6 = z[6] - g * (0.8 * (J->dn0 * w[0] + J->dn1 * w[1]));
+  const double r7 = z[7] + g * (((J->nl0 + J->ns0) * w[0] + (J->nl1 + J->ns1) * w[1]) + (J->nl2 * w[2] + J->ns3 * w[3]));
+  const double r8 = z[8] + g * (J->an0 * w[0] + J->an1 * w[1]);
+  const double A55 = 1.0 + g * ((J->ox5 + 2.0 * J->ni5) + J->od5), A56 = g * J->ox6;
+  const double A57 = g * (2.0 * J->ni7), A58 = g * J->od8;
+  const double A65 = 0.0 - g * (J->ni5 - 0.8 * J->dn5), A66 = 1.0 + g * (0.8 * J->dn6), A67 = 0.0 - g * J->ni7;
+  const double A75 = g * J->ni5, A77 = 1.0 + g * J->ni7;
+  const double A85 = 0.0 - g * (J->an5 - J->od5), A86 = 0.0 - g * J->an6, A88 = 1.0 + g * J->od8;
+  const double i77 = 1.0 / A77, i88 = 1.0 / A88;
+  const double m75 = A75 * i77, r7s = r7 * i77, r8s = r8 * i88;
+  const double b55 = (A55 - A57 * m75) - A58 * (A85 * i88);
+  const double b56 = A56 - A58 * (A86 * i88);
+  const double b65 = A65 - A67 * m75;
+  const double rr5 = (r5 - A57 * r7s) - A58 * r8s;
+  const double rr6 = r6 - A67 * r7s;
+  const double det2 = b55 * A66 - b56 * b65;
+  if (!(det2 > 0.0)) return 1;
+  const double id2 = 1.0 / det2;
+  w[5] = (A66 * rr5 - b56 * rr6) * id2;
+  w[6] = (b55 * rr6 - b65 * rr5) * id2;
+  w[7] = (r7 - A75 * w[5]) * i77;
+  w[8] = ((r8 - A85 * w[5]) - A86 * w[6]) * i88;
+  const double r9 = z[9] + g * ((((J->pp0 * w[0] + J->pp1 * w[1]) + J->pp4 * w[4]) - J->pa5 * w[5]) + J->pd8 * w[8]);
+  const double r10 = z[10] + g * (J->pa5 * w[5] - J->pd8 * w[8]);
+  const double A99 = 1.0 + g * J->pa9, A9a = 0.0 - g * J->pd10, Aa9 = 0.0 - g * J->pa9, Aaa = 1.0 + g * J->pd10;
+  const double det3 = A99 * Aaa - A9a * Aa9;
+  if (!(det3 > 0.0)) return 1;
+  const double id3 = 1.0 / det3;
+  w[9] = (Aaa * r9 - A9a * r10) * id3;
+  w[10] = (A99 * r10 - Aa9 * r9) * id3;
+  return 0;
+}
+
+/* For the tests: Jz = J_R z, w = (I - g J_R)^-1 z and gs = (grad Cprod . z,
+ * grad 0.8 Denit . z) at state c; returns 1 if the solve fails. */
+int oc_kin_jac_ops(const oc_params *p, double fT, const double *c, double g, const double *z, double *Jz,
+                   double *w, double *gs)
+{
+  oc_kjac J;
+  oc_kin_jac(p, fT, c, &J);
+  kjac_apply(&J, z, Jz);
+  gs[0] = kjac_sinkC(&J, z);
+  gs[1] = kjac_sinkN(&J, z);
+  return kjac_solve(&J, g, z, w);
+}
+
+/* per-species tridiagonal factor B = I - g J_T: sub-diagonal -a_k,
+ * diagonal b_k, super-diagonal -c_k with
+ *   a_k = g W_k G_k q_k-1,  c_k = g W_k G_k+1 q_k+1,
+ *   b_k = 1 + g W_k (G_k [k > 0 or Dirichlet top] + G_k+1 [k < K-1]) q_k,
+ * q = phi (particulate) or 1; Thomas elimination, inv_k = 1/pivot_k */
+static inline double tri_a(double g, const double *G, const double *W, const double *q, int k)
+{
+  return k > 0 ? g * ((W[k] * G[k]) * q[k - 1]) : 0.0;
+}
+static inline double tri_c(double g, const double *G, const double *W, const double *q, int k, int K)
+{
+  return k < K - 1 ? g * ((W[k] * G[k + 1]) * q[k + 1]) : 0.0;
+}
+static void tri_factor(double g, const double *G, const double *W, const double *q, int K, int dir, double *inv)
+{
+  double e = 0.0;
+  for (int k = 0; k < K; ++k) {
+    const double s = ((k > 0 || dir) ? G[k] : 0.0) + (k < K - 1 ? G[k + 1] : 0.0);
+    const double b = 1.0 + g * (W[k] * (s * q[k]));
+    const double piv = k > 0 ? b - tri_a(g, G, W, q, k) * e : b;
+    inv[k] = 1.0 / piv;
+    e = tri_c(g, G, W, q, k, K) * inv[k];
+  }
+}
+static void tri_solve(double g, const double *G, const double *W, const double *q, int K, const double *inv,
+                      const double *r, double *z)
+{
+  double d = 0.0;
+  for (int k = 0; k < K; ++k) {
+    d = k > 0 ? (r[k] + tri_a(g, G, W, q, k) * d) * inv[k] : r[k] * inv[k];
+    z[k] = d;
+  }
+  for (int k = K - 2; k >= 0; --k) z[k] = z[k] + (tri_c(g, G, W, q, k, K) * inv[k]) * z[k + 1];
+}
+/* out = J_T z for one species: the interface fluxes of z, with the
+ * linearised Dirichlet top flux -G_0 q_0 z_0 when dir, zero at the bottom */
+static void tri_apply(const double *G, const double *W, const double *q, int K, int dir, const double *z, double *o)
+{
+  double Fz = dir ? -(G[0] * (z[0] * q[0])) : 0.0;  /* flux through the upper interface of layer k */
+  for (int k = 0; k < K; ++k) {
+    const double Fd = k < K - 1 ? -(G[k + 1] * (z[k + 1] * q[k + 1] - z[k] * q[k])) : 0.0;
+    o[k] = (Fz - Fd) * W[k];
+    Fz = Fd;
+  }
+}
+
+/* Dirichlet top boundary for variable n */
+static inline int ros_dir(const oc_params *p, int n) { return (oc_part[n] ? p->bcup_part : p->bcup_dis) == 2; }
+
+typedef struct {
+  double Gb[OC_KMAX], Gd[OC_KMAX], W[OC_KMAX], one[OC_KMAX];
+  oc_kjac J[OC_KMAX];
+  double inv[NV][OC_KMAX];
+  double v[NV][OC_KMAX], k0[NV][OC_KMAX], t[NV][OC_KMAX];
+  double f0[NV][OC_KMAX], f1[NV][OC_KMAX], k1[NV][OC_KMAX], k2[NV][OC_KMAX], c1[NV][OC_KMAX];
+  double s[2][2][OC_KMAX], F[2][NV], top[2][NV];
+} roswork;
+
+/* k = W^-1 r (AMF and one correction); top[n] receives the linearised
+ * Dirichlet top-flux terms -G_0 q_0 (k0 + v2) of the two transport solves
+ * (0 for other species).  Returns 1 on a failed local solve. */
+static int ros_wsolve(const oc_params *p, const colview *v, roswork *w, double g, double (*r)[OC_KMAX],
+                      double (*k)[OC_KMAX], double *top)
+{
+  const int K = v->K;
+  double a[NV], b[NV];
+  for (int n = 0; n < NV; ++n) {
+    const double *G = oc_part[n] ? w->Gb : w->Gd, *q = oc_part[n] ? v->phi : w->one;
+    tri_solve(g, G, w->W, q, K, w->inv[n], r[n], w->v[n]);
+  }
+  for (int l = 0; l < K; ++l) {
+    for (int n = 0; n < NV; ++n) a[n] = w->v[n][l];
+    if (kjac_solve(&w->J[l], g, a, b)) return 1;
+    for (int n = 0; n < NV; ++n) w->k0[n][l] = b[n];
+  }
+  for (int n = 0; n < NV; ++n) top[n] = w->k0[n][0];
+  /* residual r - (I - g (J_T + J_R)) k0 = (r - k0) + g (J_T k0 + J_R k0) */
+  for (int n = 0; n < NV; ++n) {
+    const double *G = oc_part[n] ? w->Gb : w->Gd, *q = oc_part[n] ? v->phi : w->one;
+    tri_apply(G, w->W, q, K, ros_dir(p, n), w->k0[n], w->t[n]);
+  }
+  for (int l = 0; l < K; ++l) {
+    for (int n = 0; n < NV; ++n) a[n] = w->k0[n][l];
+    kjac_apply(&w->J[l], a, b);
+    for (int n = 0; n < NV; ++n) w->t[n][l] = (r[n][l] - w->k0[n][l]) + g * (w->t[n][l] + b[n]);
+  }
+  for (int n = 0; n < NV; ++n) {
+    const double *G = oc_part[n] ? w->Gb : w->Gd, *q = oc_part[n] ? v->phi : w->one;
+    tri_solve(g, G, w->W, q, K, w->inv[n], w->t[n], w->v[n]);
+    top[n] = ros_dir(p, n) ? 0.0 - ((G[0] * q[0]) * (top[n] + w->v[n][0])) : 0.0;
+  }
+  for (int l = 0; l < K; ++l) {
+    for (int n = 0; n < NV; ++n) a[n] = w->v[n][l];
+    if (kjac_solve(&w->J[l], g, a, b)) return 1;
+    for (int n = 0; n < NV; ++n) k[n][l] = w->k0[n][l] + b[n];
+  }
+  return 0;
+}
+
+/* Adaptive ROS2 over t_span for one column (same interface, controller and
+ * budget layout as col_emp_adaptive).  Budget: the top fluxes integrate as
+ *   h/2 [(F(c) + F(c + h k1)) + g (T_1 + T_2)]
+ * with T_i the linearised Dirichlet top-flux terms of stage i's solve, and
+ * the per-cell sinks as
+ *   h/2 [(s(c) + s(c + h k1)) + g (grad s . k1 + grad s . k2)]
+ * for s = Cprod (DIC) and 0.8 Denit (N2): exactly what the update moves,
+ * since B changes a column inventory only through its top flux and A only
+ * through these sinks.
+ * Diagnostics: zero_sink counts rejections for negativity, newton rejected
+ * local solves. */
+static int col_ros2_adaptive(const oc_params *p, const colview *v, double *c, long ks, double t_span,
+                             double *F1out, double *cellacc, long ncell, double *colacc, long ncol,
+                             oc_coldiag *dg, int64_t max_attempts, roswork *w)
+{
+  const int K = v->K;
+  const long S = OC_KMAX;
+  if (v->zw || p->bcup_part == 4) { dg->status = 3; return 3; }
+  {
+    double B = v->B, fT = oc_fT(v->T);
+    for (int k = 0; k < K; ++k) {
+      const double Dbio = ((((B * fT) / 86400.0) / 10000.0) * (1.0 - v->phiI[k])) * v->bfac[k];
+      const double Ddis = Dbio + (((p->D0 + v->T * 0.035) * v->phiI[k]) / 86400.0) / 10000.0;
+      const double len = k == 0 ? v->dz[k] : v->dzc[k];
+      w->Gb[k] = Dbio / len;
+      w->Gd[k] = Ddis / len;
+      w->W[k] = 1.0 / (v->phi[k] * v->dz[k]);
+      w->one[k] = 1.0;
+    }
+  }
+  const double fT = oc_fT(v->T);
+  const double negtol = 1.0e-3 * p->emp_atol;
+  double rem = t_span;
+  double dt = dg->dt_next;
+  if (!(dt > 0.0)) dt = p->emp_dt_max;
+  int64_t attempts = 0;
+  while (rem > 0.0) {
+    if (attempts >= max_attempts) { dg->status = 2; break; }
+    ++attempts;
+    int clipped = !(dt < rem);
+    double h = clipped ? rem : dt;
+    const double g = ROS_GAMMA * h;
+    col_rhs(p, v, c, ks, &w->f0[0][0], S, w->F[0], w->s[0][0], w->s[0][1]);
+    for (int l = 0; l < K; ++l) {
+      double a[NV];
+      for (int n = 0; n < NV; ++n) a[n] = c[n * ks + l];
+      oc_kin_jac(p, fT, a, &w->J[l]);
+    }
+    for (int n = 0; n < NV; ++n) {
+      const double *G = oc_part[n] ? w->Gb : w->Gd, *q = oc_part[n] ? v->phi : w->one;
+      tri_factor(g, G, w->W, q, K, ros_dir(p, n), w->inv[n]);
+    }
+    int fail = ros_wsolve(p, v, w, g, w->f0, w->k1, w->top[0]);
+    if (!fail) {
+      for (int n = 0; n < NV; ++n)
+        for (int l = 0; l < K; ++l) {
+          double y = c[n * ks + l] + h * w->k1[n][l];
+          w->c1[n][l] = y < 0.0 ? 0.0 : y;
+        }
+      col_rhs(p, v, &w->c1[0][0], S, &w->f1[0][0], S, w->F[1], w->s[1][0], w->s[1][1]);
+      for (int n = 0; n < NV; ++n)
+        for (int l = 0; l < K; ++l) w->f1[n][l] = w->f1[n][l] - 2.0 * w->k1[n][l];
+      fail = ros_wsolve(p, v, w, g, w->f1, w->k2, w->top[1]);
+    }
+    double err = 0.0;
+    int bad = 0, neg = 0;
+    if (!fail) {
+      for (int l = 0; l < K; ++l) {
+        double bn = 0.0, bd = 1.0;
+        for (int n = 0; n < NV; ++n) {
+          const double k1 = w->k1[n][l], k2 = w->k2[n][l], c0 = c[n * ks + l];
+          const double y = c0 + h * (1.5 * k1 + 0.5 * k2);
+          w->c1[n][l] = y;
+          const double yp = y < 0.0 ? 0.0 : y;
+          const double sc = p->emp_atol + p->emp_rtol * (yp > c0 ? yp : c0);
+          const double dv = fabs(h * (0.5 * k1 + 0.5 * k2));
+          if (dv != dv || sc != sc) bad = 1;
+          if (y < 0.0 - negtol) neg = 1;
+          if (dv * bd > bn * sc) { bn = dv; bd = sc; }
+        }
+        const double e = bn / bd;
+        if (e != e) bad = 1;
+        if (e > err) err = e;
+      }
+    }
+    if (bad) { dg->status = 1; break; }
+    if (fail && !(h > p->emp_dt_min)) { dg->status = 1; break; }
+    const int accept = !fail && ((err <= 1.0 && !neg) || !(h > p->emp_dt_min));
+    double fac = err > 0.0 ? 0.9 * oc_rsqrt(err) : 5.0;
+    if (fac < 0.2) fac = 0.2;
+    if (fac > 5.0) fac = 5.0;
+    if (accept) {
+      for (int n = 0; n < NV; ++n)
+        for (int l = 0; l < K; ++l) {
+          double y = w->c1[n][l];
+          if (y < 0.0) {
+            if (cellacc && oc_elem[n] >= 0) ACC_CELL(cellacc, ncell, 2 + oc_elem[n], l, 0.0 - y);
+            y = 0.0;
+          }
+          c[n * ks + l] = y;
+        }
+      if (colacc)
+        for (int n = 0; n < NV; ++n)
+          ACC_COL(colacc, ncol, n, (h * 0.5) * ((w->F[0][n] + w->F[1][n]) + g * (w->top[0][n] + w->top[1][n])));
+      if (cellacc)
+        for (int l = 0; l < K; ++l) {
+          double a1[NV], a2[NV];
+          for (int n = 0; n < NV; ++n) { a1[n] = w->k1[n][l]; a2[n] = w->k2[n][l]; }
+          const double gC = kjac_sinkC(&w->J[l], a1) + kjac_sinkC(&w->J[l], a2);
+          const double gN = kjac_sinkN(&w->J[l], a1) + kjac_sinkN(&w->J[l], a2);
+          ACC_CELL(cellacc, ncell, 0, l, (h * 0.5) * ((w->s[0][0][l] + w->s[1][0][l]) + g * gC));
+          ACC_CELL(cellacc, ncell, 1, l, (h * 0.5) * ((w->s[0][1][l] + w->s[1][1][l]) + g * gN));
+        }
+      memcpy(F1out, w->F[1], sizeof(double) * NV);
+      rem = rem - h;
+      dg->accepted += 1;
+      if (err > 1.0 || neg) dg->forced += 1;
+      dg->dt_last = h;
+      if (h < dg->dt_minacc) dg->dt_minacc = h;
+      double prop = h * fac;
+      dt = (clipped && dt > prop) ? dt : prop;
+    } else {
+      dg->rejected += 1;
+      if (fail) {
+        dg->newton += 1;
+        dt = h * 0.25;
+      } else if (err <= 1.0) {
+        dg->zero_sink += 1;
+        dt = h * 0.5;
+      } else {
+        dt = h * fac;
+      }
+    }
+    if (dt < p->emp_dt_min) dt = p->emp_dt_min;
+    if (dt > p->emp_dt_max) dt = p->emp_dt_max;
+  }
+  dg->dt_next = dt;
+  return dg->status;
+}
+
+/* ------------------------------------------------------------------------ */
 /* domain-level entry points                                                 */
 /* ------------------------------------------------------------------------ */
 
@@ -1070,6 +1508,12 @@ int oc_ode_solver(const oc_params *p, const oc_domain *d, double *conc,
     if (method == OC_EMP_ADAPTIVE)
       r = col_emp_adaptive(p, &v, conc + o, ncell, dt, F1, cellacc ? cellacc + o : NULL, ncell,
                            colacc ? colacc + c : NULL, ncol, dg, INT64_MAX, w);
+    else if (method == OC_ROS2_ADAPTIVE) {
+      roswork *rw = (roswork *)malloc(sizeof(roswork));
+      r = col_ros2_adaptive(p, &v, conc + o, ncell, dt, F1, cellacc ? cellacc + o : NULL, ncell,
+                            colacc ? colacc + c : NULL, ncol, dg, INT64_MAX, rw);
+      free(rw);
+    }
     else
       r = col_step(p, &v, method, dt, conc + o, ncell, F1, cellacc ? cellacc + o : NULL, ncell,
                    colacc ? colacc + c : NULL, ncol, dg, w);
@@ -1145,6 +1589,7 @@ int oc_emp_integrate(const oc_params *p, const oc_domain *d, double *conc,
 #endif
   {
     colwork *w = (colwork *)malloc(sizeof(colwork));
+    roswork *rw = p->adaptive_method == OC_ROS2_ADAPTIVE ? (roswork *)malloc(sizeof(roswork)) : NULL;
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 1)
 #endif
@@ -1156,12 +1601,15 @@ int oc_emp_integrate(const oc_params *p, const oc_domain *d, double *conc,
       double F1[NV];
       for (int n = 0; n < NV; ++n) F1[n] = flux_out ? flux_out[(long)n * ncol + c] : 0.0;
       diag_init(&diag[c]);
-      int r = col_emp_adaptive(p, &v, conc + o, ncell, t_span, F1, cellacc ? cellacc + o : NULL,
-                               ncell, colacc ? colacc + c : NULL, ncol, &diag[c], max_attempts, w);
+      int r = rw ? col_ros2_adaptive(p, &v, conc + o, ncell, t_span, F1, cellacc ? cellacc + o : NULL,
+                                     ncell, colacc ? colacc + c : NULL, ncol, &diag[c], max_attempts, rw)
+                 : col_emp_adaptive(p, &v, conc + o, ncell, t_span, F1, cellacc ? cellacc + o : NULL,
+                                    ncell, colacc ? colacc + c : NULL, ncol, &diag[c], max_attempts, w);
       if (flux_out) for (int n = 0; n < NV; ++n) flux_out[(long)n * ncol + c] = F1[n];
       if (r) rc |= r;
     }
     free(w);
+    free(rw);
   }
   return rc;
 }

@@ -43,7 +43,7 @@ enum {
 /* methods: 0..3 as solver_library.F90:32-35, 4..6 new */
 enum {
   OC_EULER = 0, OC_RK4 = 1, OC_ADAPTIVE_EULER = 2, OC_RK4_38 = 3,
-  OC_EMP1 = 4, OC_EMP2 = 5, OC_EMP_ADAPTIVE = 6
+  OC_EMP1 = 4, OC_EMP2 = 5, OC_EMP_ADAPTIVE = 6, OC_ROS2_ADAPTIVE = 7
 };
 
 typedef struct {
@@ -64,6 +64,8 @@ typedef struct {
   int32_t clamp;        /* clamp to minimum (0) after every step */
   int32_t arith;        /* 0 reference operation order, 1 fast form */
   int32_t emp_weights;  /* Patankar factor: 0 product, 1 geometric mean */
+  int32_t adaptive_method; /* stepper of oc_emp_integrate: 0 or 6 adaptive
+                              EMP, 7 linearly implicit ROS2 */
 } oc_params;
 
 /* Bioturbation profile 3 (Zhang & Wirtz 2017; fabm_sediment_driver.F90:
@@ -116,6 +118,10 @@ void oc_zw_geometry(const oc_zw *zw, const double *dz, int K, double *e1, double
 double oc_rsqrt(double x);
 void oc_kinetics(const oc_params *p, double fT, const double *c, double *R,
                  double *sinkC, double *sinkN);
+/* tests only: J_R z, (I - g J_R)^-1 z and the C / N sink gradients applied
+ * to z, for the Jacobian of the local rates used by method 7 */
+int oc_kin_jac_ops(const oc_params *p, double fT, const double *c, double g, const double *z, double *Jz,
+                   double *w, double *gs);
 /* tests only: select a planted mistake in the kinetics (0 = none) */
 int oc_set_plant(int which);
 int oc_rhs(const oc_params *p, const oc_domain *d, const double *conc,

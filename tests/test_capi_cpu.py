@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_sizes():
-    assert ctypes.sizeof(capi.Params) == 21 * 8 + 6 * 4
+    assert ctypes.sizeof(capi.Params) == 21 * 8 + 7 * 4 + 4  # 7 int32 fields, 4 bytes of tail padding
     assert ctypes.sizeof(capi.ColDiag) == 9 * 8 + 2 * 4
     assert ctypes.sizeof(capi.Desc) == 4 * 4 + 2 * 8
     assert ctypes.sizeof(capi.Schedule) == 4 * 4
