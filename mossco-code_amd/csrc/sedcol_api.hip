@@ -393,6 +393,9 @@ int capture_graph(sedcol_handle *h)
 // handle's schedule.  Sets *launches to the adaptive kernel launches made.
 int run_adaptive(sedcol_handle *h, double t_start, double t_end, int only, int *launches)
 {
+  if (h->p.adaptive_method == SEDCOL_ROS2_ADAPTIVE && (h->zw_on || h->p.bcup_part == 4))
+    return fail(SEDCOL_EUNSUPPORTED,
+                "the linearly implicit stepper does not support bioturbation profile 3 or bcup_part = 4");
   const int r0 = record_of(h, t_start), r1 = record_of(h, t_end);
   const sedcol_schedule &sc_ = h->sched;
   const bool windowed = sc_.window > 0;
@@ -639,6 +642,9 @@ int sedcol_set_params(sedcol_handle *h, const sedcol_params *p)
   if (p->arith != SEDCOL_ARITH_REFERENCE && p->arith != SEDCOL_ARITH_FAST)
     return fail(SEDCOL_EINVAL, "arith must be SEDCOL_ARITH_REFERENCE or SEDCOL_ARITH_FAST");
   if (p->emp_weights != 0 && p->emp_weights != 1) return fail(SEDCOL_EINVAL, "emp_weights must be 0 or 1");
+  if (p->adaptive_method != 0 && p->adaptive_method != SEDCOL_EMP_ADAPTIVE &&
+      p->adaptive_method != SEDCOL_ROS2_ADAPTIVE)
+    return fail(SEDCOL_EINVAL, "adaptive_method must be 0, SEDCOL_EMP_ADAPTIVE or SEDCOL_ROS2_ADAPTIVE");
   h->p = *p;
   h->user_arith = p->arith;
   if (h->zw_on) h->p.arith = sc::ARITH_EXT;
@@ -1230,11 +1236,23 @@ int sedcol_step(sedcol_handle *h, double dt, int32_t method, double *conc_inout,
   if (rc) return rc;
   const DeviceScope dscope(h->device);
   if (!conc_inout) return fail(SEDCOL_EINVAL, "null argument");
-  if (method < SEDCOL_EULER || method > SEDCOL_EMP_ADAPTIVE) return fail(SEDCOL_EINVAL, "unknown method");
+  if (method < SEDCOL_EULER || method > SEDCOL_ROS2_ADAPTIVE) return fail(SEDCOL_EINVAL, "unknown method");
   if ((rc = sedcol_set_state(h, conc_inout))) return rc;
-  if (method == SEDCOL_EMP_ADAPTIVE) {
+  if (method == SEDCOL_EMP_ADAPTIVE || method == SEDCOL_ROS2_ADAPTIVE) {
     if (h->nrec > 1) return fail(SEDCOL_EINVAL, "sedcol_step uses a single forcing record");
+    // the adaptive stepper of this call; the handle's choice is restored
+    const int32_t keep = h->p.adaptive_method;
+    const int32_t want = method == SEDCOL_ROS2_ADAPTIVE ? SEDCOL_ROS2_ADAPTIVE : 0;
+    const bool swap = (keep == SEDCOL_ROS2_ADAPTIVE) != (want == SEDCOL_ROS2_ADAPTIVE);
+    if (swap) {
+      h->p.adaptive_method = want;
+      drop_graph(h);
+    }
     rc = sedcol_integrate(h, 0.0, dt);
+    if (swap) {
+      h->p.adaptive_method = keep;
+      drop_graph(h);
+    }
   } else {
     rc = sedcol_run(h, dt, 1, method, 0);
   }

@@ -228,6 +228,175 @@ __device__ __forceinline__ void kinetics(const sedcol_params &p, double fT, cons
 }
 
 // ---------------------------------------------------------------------------
+// Jacobian of the local rates (linearly implicit stepper, method 7)
+// ---------------------------------------------------------------------------
+
+// Nonzero partial derivatives of the processes of kinetics<> with respect to
+// the state, at max(c, 0), from the reference-form expressions (the W
+// matrix of the stepper needs no more than that; both arithmetic forms use
+// it).  Same operation sequence as the oracle's oc_kin_jac.
+struct KJac {
+  double cl0, cl1, cs0, cs1;  // dCpL/d(ldC,sdC), dCpS/d(ldC,sdC)
+  double nl0, nl1, nl2;       // dNpL/d(ldC,sdC,ldN)
+  double ns0, ns1, ns3;       // dNpS/d(ldC,sdC,sdN)
+  double pp0, pp1, pp4;       // dPprod/d(ldC,sdC,detP)
+  double ox0, ox1, ox5, ox6;  // dOxicMin/d(ldC,sdC,O2,NO3)
+  double dn0, dn1, dn5, dn6;  // dDenit
+  double an0, an1, an5, an6;  // dAnoxMin
+  double ni5, ni7;            // dNitri/d(O2,NH4)
+  double od5, od8;            // dOduOx/d(O2,ODU)
+  double pa5, pa9;            // dPads/d(O2,PO4)
+  double pd8, pd10;           // dPdes/d(ODU,adsP)
+  double gc0, gc1;            // dCprod/d(ldC,sdC)
+};
+
+__device__ __forceinline__ void kin_jac(const sedcol_params &p, double fT, const double (&c)[NV], KJac &J)
+{
+  const double ldC = nn(c[0]), sdC = nn(c[1]), ldN = nn(c[2]), sdN = nn(c[3]);
+  const double dP = nn(c[4]), O2 = nn(c[5]), NO3 = nn(c[6]), NH4 = nn(c[7]);
+  const double ODU = nn(c[8]), PO4 = nn(c[9]), aP = nn(c[10]);
+  const double kL = fT * p.rL, kS = fT * p.rS;
+  const double dL = kL * ldC, dS = kS * sdC;
+  const double den = p.CprodMax + (dL + dS);
+  const double lim = p.CprodMax / den;
+  const double q = lim / den;
+  J.cl0 = kL * (lim - dL * q);
+  J.cl1 = -((dL * kS) * q);
+  J.cs0 = -((dS * kL) * q);
+  J.cs1 = kS * (lim - dS * q);
+  J.gc0 = J.cl0 + J.cs0;
+  J.gc1 = J.cl1 + J.cs1;
+  const double aN = kL * ldN, bN = kS * sdN, aP4 = kL * dP;
+  J.nl0 = -((aN * kL) * q);
+  J.nl1 = -((aN * kS) * q);
+  J.nl2 = kL * lim;
+  J.ns0 = -((bN * kL) * q);
+  J.ns1 = -((bN * kS) * q);
+  J.ns3 = kS * lim;
+  J.pp0 = -((aP4 * kL) * q);
+  J.pp1 = -((aP4 * kS) * q);
+  J.pp4 = kL * lim;
+  const double Cprod = dL * lim + dS * lim;
+  const double qo = O2 + p.ksO2oxic;
+  const double oxl = O2 / qo;
+  const double oxl5 = p.ksO2oxic / (qo * qo);
+  const double qd = O2 + p.kinO2denit, qn = NO3 + p.ksNO3denit;
+  const double wd = p.kinO2denit / qd, mn = NO3 / qn;
+  const double dnl = wd * mn;
+  const double dnl5 = -(dnl / qd);
+  const double dnl6 = wd * (p.ksNO3denit / (qn * qn));
+  const double qa = O2 + p.kinO2anox, qb = NO3 + p.kinNO3anox;
+  const double anl = (p.kinO2anox / qa) * (p.kinNO3anox / qb);
+  const double anl5 = -(anl / qa);
+  const double anl6 = -(anl / qb);
+  const double rs = 1.0 / ((oxl + dnl) + anl);
+  const double S5 = (oxl5 + dnl5) + anl5, S6 = dnl6 + anl6;
+  const double shox = oxl * rs, shdn = dnl * rs, shan = anl * rs;
+  J.ox0 = J.gc0 * shox;
+  J.ox1 = J.gc1 * shox;
+  J.ox5 = Cprod * ((oxl5 - shox * S5) * rs);
+  J.ox6 = Cprod * ((0.0 - shox * S6) * rs);
+  J.dn0 = J.gc0 * shdn;
+  J.dn1 = J.gc1 * shdn;
+  J.dn5 = Cprod * ((dnl5 - shdn * S5) * rs);
+  J.dn6 = Cprod * ((dnl6 - shdn * S6) * rs);
+  J.an0 = J.gc0 * shan;
+  J.an1 = J.gc1 * shan;
+  J.an5 = Cprod * ((anl5 - shan * S5) * rs);
+  J.an6 = Cprod * ((anl6 - shan * S6) * rs);
+  const double rn = fT * p.rnit, qi = O2 + p.ksO2nitri;
+  J.ni7 = rn * (O2 / qi);
+  J.ni5 = (rn * NH4) * (p.ksO2nitri / (qi * qi));
+  const double ro = fT * p.rODUox, qu = O2 + p.ksO2oduox;
+  const double o2u = O2 / qu, o2u5 = p.ksO2oduox / (qu * qu);
+  J.od8 = ro * o2u;
+  J.od5 = (ro * ODU) * o2u5;
+  const double kP = fT * p.rPads, qp = ODU + p.PAdsODU;
+  J.pa9 = kP * o2u;
+  J.pa5 = (kP * PO4) * o2u5;
+  J.pd10 = kP * (ODU / qp);
+  J.pd8 = (kP * aP) * (p.PAdsODU / (qp * qp));
+}
+
+// o = J_R z
+__device__ __forceinline__ void kjac_apply(const KJac &J, const double (&z)[NV], double (&o)[NV])
+{
+  o[0] = -(J.cl0 * z[0] + J.cl1 * z[1]);
+  o[1] = -(J.cs0 * z[0] + J.cs1 * z[1]);
+  o[2] = -((J.nl0 * z[0] + J.nl1 * z[1]) + J.nl2 * z[2]);
+  o[3] = -((J.ns0 * z[0] + J.ns1 * z[1]) + J.ns3 * z[3]);
+  o[4] = -((J.pp0 * z[0] + J.pp1 * z[1]) + J.pp4 * z[4]);
+  const double ox = (J.ox0 * z[0] + J.ox1 * z[1]) + (J.ox5 * z[5] + J.ox6 * z[6]);
+  const double dn = (J.dn0 * z[0] + J.dn1 * z[1]) + (J.dn5 * z[5] + J.dn6 * z[6]);
+  const double an = (J.an0 * z[0] + J.an1 * z[1]) + (J.an5 * z[5] + J.an6 * z[6]);
+  const double ni = J.ni5 * z[5] + J.ni7 * z[7];
+  const double od = J.od5 * z[5] + J.od8 * z[8];
+  const double pa = J.pa5 * z[5] + J.pa9 * z[9];
+  const double pd = J.pd8 * z[8] + J.pd10 * z[10];
+  o[5] = -((ox + 2.0 * ni) + od);
+  o[6] = ni - 0.8 * dn;
+  o[7] = (0.0 - (o[2] + o[3])) - ni;
+  o[8] = an - od;
+  o[9] = ((0.0 - o[4]) - pa) + pd;
+  o[10] = pa - pd;
+}
+
+__device__ __forceinline__ double kjac_sinkC(const KJac &J, const double (&z)[NV]) { return J.gc0 * z[0] + J.gc1 * z[1]; }
+__device__ __forceinline__ double kjac_sinkN(const KJac &J, const double (&z)[NV])
+{
+  return 0.8 * ((J.dn0 * z[0] + J.dn1 * z[1]) + (J.dn5 * z[5] + J.dn6 * z[6]));
+}
+
+// (I - g J_R) w = z by the block structure (detritus C 2x2, detritus N and
+// P, O2/NO3 2x2 after eliminating NH4 and ODU, PO4/adsP 2x2); false if a
+// pivot is not positive
+__device__ __forceinline__ bool kjac_solve(const KJac &J, double g, const double (&z)[NV], double (&w)[NV])
+{
+  bool ok = true;
+  const double a00 = 1.0 + g * J.cl0, a01 = g * J.cl1, a10 = g * J.cs0, a11 = 1.0 + g * J.cs1;
+  const double det = a00 * a11 - a01 * a10;
+  ok = ok && det > 0.0;
+  const double idet = 1.0 / det;
+  w[0] = (a11 * z[0] - a01 * z[1]) * idet;
+  w[1] = (a00 * z[1] - a10 * z[0]) * idet;
+  w[2] = (z[2] - g * (J.nl0 * w[0] + J.nl1 * w[1])) / (1.0 + g * J.nl2);
+  w[3] = (z[3] - g * (J.ns0 * w[0] + J.ns1 * w[1])) / (1.0 + g * J.ns3);
+  w[4] = (z[4] - g * (J.pp0 * w[0] + J.pp1 * w[1])) / (1.0 + g * J.pp4);
+  const double r5 = z[5] - g * (J.ox0 * w[0] + J.ox1 * w[1]);
+  const double r6 = z[6] - g * (0.8 * (J.dn0 * w[0] + J.dn1 * w[1]));
+  const double r7 = z[7] + g * (((J.nl0 + J.ns0) * w[0] + (J.nl1 + J.ns1) * w[1]) + (J.nl2 * w[2] + J.ns3 * w[3]));
+  const double r8 = z[8] + g * (J.an0 * w[0] + J.an1 * w[1]);
+  const double A55 = 1.0 + g * ((J.ox5 + 2.0 * J.ni5) + J.od5), A56 = g * J.ox6;
+  const double A57 = g * (2.0 * J.ni7), A58 = g * J.od8;
+  const double A65 = 0.0 - g * (J.ni5 - 0.8 * J.dn5), A66 = 1.0 + g * (0.8 * J.dn6), A67 = 0.0 - g * J.ni7;
+  const double A75 = g * J.ni5, A77 = 1.0 + g * J.ni7;
+  const double A85 = 0.0 - g * (J.an5 - J.od5), A86 = 0.0 - g * J.an6, A88 = 1.0 + g * J.od8;
+  const double i77 = 1.0 / A77, i88 = 1.0 / A88;
+  const double m75 = A75 * i77, r7s = r7 * i77, r8s = r8 * i88;
+  const double b55 = (A55 - A57 * m75) - A58 * (A85 * i88);
+  const double b56 = A56 - A58 * (A86 * i88);
+  const double b65 = A65 - A67 * m75;
+  const double rr5 = (r5 - A57 * r7s) - A58 * r8s;
+  const double rr6 = r6 - A67 * r7s;
+  const double det2 = b55 * A66 - b56 * b65;
+  ok = ok && det2 > 0.0;
+  const double id2 = 1.0 / det2;
+  w[5] = (A66 * rr5 - b56 * rr6) * id2;
+  w[6] = (b55 * rr6 - b65 * rr5) * id2;
+  w[7] = (r7 - A75 * w[5]) * i77;
+  w[8] = ((r8 - A85 * w[5]) - A86 * w[6]) * i88;
+  const double r9 = z[9] + g * ((((J.pp0 * w[0] + J.pp1 * w[1]) + J.pp4 * w[4]) - J.pa5 * w[5]) + J.pd8 * w[8]);
+  const double r10 = z[10] + g * (J.pa5 * w[5] - J.pd8 * w[8]);
+  const double A99 = 1.0 + g * J.pa9, A9a = 0.0 - g * J.pd10, Aa9 = 0.0 - g * J.pa9, Aaa = 1.0 + g * J.pd10;
+  const double det3 = A99 * Aaa - A9a * Aa9;
+  ok = ok && det3 > 0.0;
+  const double id3 = 1.0 / det3;
+  w[9] = (Aaa * r9 - A9a * r10) * id3;
+  w[10] = (A99 * r10 - Aa9 * r9) * id3;
+  return ok;
+}
+
+// ---------------------------------------------------------------------------
 // lane exchange
 // ---------------------------------------------------------------------------
 

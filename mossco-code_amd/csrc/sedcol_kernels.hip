@@ -587,6 +587,435 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
 }
 
 // ---------------------------------------------------------------------------
+// linearly implicit adaptive stepper (method 7): ROS2 as a W-method
+// ---------------------------------------------------------------------------
+//
+//   W k1 = f(c),  W k2 = f(c + h k1) - 2 k1,  c+ = c + h (3/2 k1 + 1/2 k2),
+//   error estimate h (k1 + k2)/2 against the first-order c + h k1,
+// gamma = 1 + 1/sqrt(2) (Verwer, Spee, Blom and Hundsdorfer 1999).  W is
+// the product of the per-species tridiagonal transport factor
+// B = I - gamma h J_T (Thomas sweeps, one lane per species, through LDS)
+// and the per-cell reaction factor A = I - gamma h J_R (block solve in the
+// lane's registers), plus one correction with the full matrix:
+//   k0 = A^-1 B^-1 r,  k = k0 + A^-1 B^-1 (r - (I - gamma h (J_T + J_R)) k0).
+// The transport coefficients come from the same diffusivities as the
+// right-hand side, whose arithmetic form (reference or fast) is the
+// instance's.  Budgets: top fluxes integrate as h/2 [(F0 + F1) + g (T1 +
+// T2)] with T the linearised Dirichlet top-flux terms of the two solves,
+// the DIC and N2 sinks as h/2 [(s0 + s1) + g grad s . (k1 + k2)]: exactly
+// what the update moves.  Positivity: a new state below -1e-3 atol
+// rejects the step (zero_sink counts these); smaller negative values are
+// clamped to 0 and booked as clamp mass.  Failed local solves (a pivot
+// that is not positive) reject the step with a quarter step (newton counts
+// these).  Same operation sequence as the oracle's col_ros2_adaptive.
+
+constexpr double ROS_GAMMA = 1.7071067811865475;
+
+// per-group LDS of the stepper
+template <int LPC>
+struct RosShared {
+  double G[2][LPC];     // transport coefficients D/dzc (D/dz at the top): [0] particulate, [1] dissolved
+  double W[LPC];        // 1/(phi dz)
+  double ph[LPC];       // porosity
+  double T[NV][LPC];    // right-hand sides and solutions of the species sweeps
+  double inv[NV][LPC];  // reciprocal Thomas pivots
+  double top[2][NV];    // linearised Dirichlet top-flux terms of the two stages
+};
+
+// Thomas factor of species n = l (lanes l < NV of the group)
+template <int LPC>
+__device__ __forceinline__ void ros_factor(RosShared<LPC> &R, int l, int K, double g, unsigned dirmask)
+{
+  if (l < NV) {
+    const bool pt = part(l);
+    const double *G = R.G[pt ? 0 : 1];
+    const bool dir = (dirmask >> l) & 1u;
+    double e = 0.0;
+    for (int k = 0; k < K; ++k) {
+      const double qk = pt ? R.ph[k] : 1.0;
+      const double s = ((k > 0 || dir) ? G[k] : 0.0) + (k < K - 1 ? G[k + 1] : 0.0);
+      const double b = 1.0 + g * (R.W[k] * (s * qk));
+      const double a = k > 0 ? g * ((R.W[k] * G[k]) * (pt ? R.ph[k - 1] : 1.0)) : 0.0;
+      const double piv = k > 0 ? b - a * e : b;
+      const double iv = 1.0 / piv;
+      R.inv[l][k] = iv;
+      const double c = k < K - 1 ? g * ((R.W[k] * G[k + 1]) * (pt ? R.ph[k + 1] : 1.0)) : 0.0;
+      e = c * iv;
+    }
+  }
+}
+
+// B z = T in place, species n = l
+template <int LPC>
+__device__ __forceinline__ void ros_sweep(RosShared<LPC> &R, int l, int K, double g)
+{
+  if (l < NV) {
+    const bool pt = part(l);
+    const double *G = R.G[pt ? 0 : 1];
+    double *T = R.T[l];
+    const double *iv = R.inv[l];
+    double d = 0.0;
+    for (int k = 0; k < K; ++k) {
+      const double a = k > 0 ? g * ((R.W[k] * G[k]) * (pt ? R.ph[k - 1] : 1.0)) : 0.0;
+      d = k > 0 ? (T[k] + a * d) * iv[k] : T[k] * iv[k];
+      T[k] = d;
+    }
+    for (int k = K - 2; k >= 0; --k) {
+      const double c = g * ((R.W[k] * G[k + 1]) * (pt ? R.ph[k + 1] : 1.0));
+      T[k] = T[k] + (c * iv[k]) * T[k + 1];
+    }
+  }
+}
+
+// B^-1 r for the lane's layer: stage through LDS, sweep, read back
+template <int LPC>
+__device__ __forceinline__ void ros_bsolve(RosShared<LPC> &R, int l, int K, bool act, double g, const double (&r)[NV],
+                                           double (&v)[NV])
+{
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int n = 0; n < NV; ++n)
+    if (act) R.T[n][l] = r[n];
+  __builtin_amdgcn_wave_barrier();
+  ros_sweep<LPC>(R, l, K, g);
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int n = 0; n < NV; ++n) v[n] = act ? R.T[n][l] : 0.0;
+  __builtin_amdgcn_wave_barrier();
+}
+
+// k = W^-1 r (AMF and one correction); r is overwritten.  Lane 0 stores
+// the stage's linearised Dirichlet top-flux terms in top[n].  Returns the
+// lane's local solve status.
+template <int LPC>
+__device__ __forceinline__ bool ros_wsolve(RosShared<LPC> &R, const KJac &J, int l, int K, bool act, double g,
+                                           unsigned dirmask, double phi, double (&r)[NV], double (&k)[NV],
+                                           double *top)
+{
+  // the lane's transport coefficients, read from the group's LDS
+  const double W = R.W[l];
+  const double Gn_p = l + 1 < K ? R.G[0][l + 1] : 0.0, Gn_d = l + 1 < K ? R.G[1][l + 1] : 0.0;
+  const double Gtop_p = R.G[0][0], Gtop_d = R.G[1][0];
+  double a[NV];
+  ros_bsolve<LPC>(R, l, K, act, g, r, a);
+  bool ok = kjac_solve(J, g, a, k);  // k = k0
+  kjac_apply(J, k, a);               // a = J_R k0
+  // residual (r - k0) + g (J_T k0 + J_R k0), into r
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    const bool pt = part(n);
+    const double q = pt ? phi : 1.0;
+    const double zp = k[n] * q;
+    const double zpn = lg_dn(zp);
+    const double Gn = pt ? Gn_p : Gn_d;
+    const double Fd = l < K - 1 ? -(Gn * (zpn - zp)) : 0.0;
+    double Fu = lg_up(Fd);
+    if (l == 0) Fu = ((dirmask >> n) & 1u) ? -((pt ? Gtop_p : Gtop_d) * zp) : 0.0;
+    const double t = (Fu - Fd) * W;
+    r[n] = (r[n] - k[n]) + g * (t + a[n]);
+  }
+  ros_bsolve<LPC>(R, l, K, act, g, r, a);  // a = v2
+  if (l == 0) {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      const bool pt = part(n);
+      top[n] = ((dirmask >> n) & 1u) ? 0.0 - (((pt ? Gtop_p : Gtop_d) * (pt ? phi : 1.0)) * (k[n] + a[n])) : 0.0;
+    }
+  }
+  ok = kjac_solve(J, g, a, r) && ok;  // r = kc
+#pragma unroll
+  for (int n = 0; n < NV; ++n) k[n] = k[n] + r[n];
+  return ok || !act;
+}
+
+// one wave per SIMD: the stepper's live state (state, two stage vectors,
+// the reaction Jacobian) exceeds the 256 registers of two waves; with one
+// wave the overflow goes to the accumulation registers instead of scratch
+#ifndef SEDCOL_ROS_MIN_WAVES
+#define SEDCOL_ROS_MIN_WAVES 1
+#endif
+
+template <int LPC, int ARITH, int BLK>
+__global__ __launch_bounds__(BLK, SEDCOL_ROS_MIN_WAVES) void k_ros_adaptive(
+    const Dev *dptr, const int *list, int nlist, const sedcol_params *__restrict__ pptr,
+    const EmpCtl *__restrict__ cptr, int *next, int *pending, int *work, int *errflag)
+{
+  const Dev &d = *dptr;
+  const sedcol_params &p = *pptr;
+  const EmpCtl &ctl = *cptr;
+  constexpr int G = BLK / LPC;
+  __shared__ double sbc[G][2 * NV];
+  __shared__ double sF[G][2][NV];
+  __shared__ int sslot[G];
+  __shared__ RosShared<LPC> sR[G];
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const bool own = l < NV;
+  double(*F)[NV] = sF[g];
+  const double *bc = sbc[g];
+  RosShared<LPC> &R = sR[g];
+  const int r1 = ctl.r1;
+  const unsigned dirmask = (p.bcup_part == 2 ? 0x41Fu : 0u) | (p.bcup_dis == 2 ? 0x3E0u : 0u);
+  const double negtol = 1.0e-3 * p.emp_atol;
+
+  int c = 0, r = 0;
+  Lane L;
+  Geo Gm;
+  long long cell = 0;
+  double x[NV];
+  __shared__ double sacc[7][BLK];
+  double &sinkC = sacc[0][threadIdx.x], &sinkN = sacc[1][threadIdx.x], &fint = sacc[2][threadIdx.x];
+  double &sinkCc = sacc[3][threadIdx.x], &sinkNc = sacc[4][threadIdx.x], &fintc = sacc[5][threadIdx.x];
+  double &flast = sacc[6][threadIdx.x];
+  sedcol_coldiag dg;
+  long long attempts = 0;
+  int wn = 0;
+  double dt = 0.0, rem = 0.0;
+  bool have = false, act = false;
+
+  auto coefficients = [&]() {
+    // W-matrix coefficients of the record (the right-hand side's
+    // diffusivities) into the group's LDS
+    double Gp, Gd, Wl;
+    if (ARITH == ARITH_FAST) {
+      Gp = L.Gb;
+      Gd = L.Gd;
+      Wl = L.W;
+    } else {
+      const double len = l == 0 ? L.dz : L.dzc;
+      Gp = L.Dbio / len;
+      Gd = L.Ddis / len;
+      Wl = 1.0 / (L.phi * L.dz);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (act) {
+      R.G[0][l] = Gp;
+      R.G[1][l] = Gd;
+      R.W[l] = Wl;
+      R.ph[l] = L.phi;
+    }
+    __builtin_amdgcn_wave_barrier();
+  };
+
+  for (;;) {
+    if (!have) {
+      if (l == 0) sslot[g] = atomicAdd(next, 1);
+      __builtin_amdgcn_wave_barrier();
+      const int s = sslot[g];
+      __builtin_amdgcn_wave_barrier();
+      if (s >= nlist) break;
+      c = list[s];
+      if (ctl.resume) {
+        r = d.wrec[c];
+        if (r > r1) continue;
+        rem = d.wrem[c];
+        attempts = d.watt[c];
+      } else {
+        r = ctl.r0 - 1;
+        rem = 0.0;
+        attempts = 0;
+      }
+      setup_lane(d, c, l, L, Gm, cell);
+      act = L.active;
+#pragma unroll
+      for (int n = 0; n < NV; ++n) x[n] = act ? d.conc[n * d.ncell + cell] : 0.0;
+      sinkC = act ? d.cellacc[cell] : 0.0;
+      sinkN = act ? d.cellacc[d.ncell + cell] : 0.0;
+      sinkCc = act ? d.cellacc[5 * d.ncell + cell] : 0.0;
+      sinkNc = act ? d.cellacc[6 * d.ncell + cell] : 0.0;
+      fint = own ? d.colacc[(long long)l * d.ncol + c] : 0.0;
+      fintc = own ? d.colacc[(long long)(NV + l) * d.ncol + c] : 0.0;
+      flast = own ? d.flux[(long long)l * d.ncol + c] : 0.0;
+      dg = d.diag[c];
+      dt = dg.dt_next;
+      if (!(dt > 0.0)) dt = p.emp_dt_max;
+      wn = 0;
+      have = true;
+      if (rem > 0.0) {
+        load_forcing<ARITH>(d, p, c, r, Gm, L, sbc[g]);
+        coefficients();
+      }
+    }
+    bool finish = false;
+    if (!(rem > 0.0)) {
+      ++r;
+      if (r > r1) {
+        finish = true;
+      } else {
+        const double lo = (r == 0) ? -INFINITY : d.t0 + (double)r * d.dt_rec;
+        const double hi = (r == d.nrec - 1) ? INFINITY : d.t0 + (double)(r + 1) * d.dt_rec;
+        const double a = ctl.t_start > lo ? ctl.t_start : lo;
+        const double b = ctl.t_end < hi ? ctl.t_end : hi;
+        rem = b - a;
+        load_forcing<ARITH>(d, p, c, r, Gm, L, sbc[g]);
+        coefficients();
+        continue;
+      }
+    }
+    bool pause = false;
+    if (!finish) {
+      if (attempts >= ctl.max_attempts) {
+        dg.status = 2;
+        if (l == 0) atomicOr(errflag, 2);
+        finish = true;
+      } else if (ctl.window > 0 && wn >= ctl.window) {
+        pause = true;
+      }
+    }
+    if (!finish && !pause) {
+      ++attempts;
+      ++wn;
+      const bool clipped = !(dt < rem);
+      const double h = clipped ? rem : dt;
+      const double gh = ROS_GAMMA * h;
+      const int K = L.K;
+      double f[NV], k1[NV], k2[NV];
+      double s0C, s0N, s1C = 0.0, s1N = 0.0;
+      column_rhs<LPC, ARITH>(p, L, bc, x, s0C, s0N, [&](int n, double rr, double Fv) {
+        f[n] = rr;
+        if (l == 0) F[0][n] = Fv;
+      });
+      __builtin_amdgcn_wave_barrier();
+      ros_factor<LPC>(R, l, K, gh, dirmask);
+      __builtin_amdgcn_wave_barrier();
+      // the Jacobian is evaluated again for the second stage (same values)
+      // rather than kept live across the second right-hand side
+      double gC = 0.0, gN = 0.0;
+      bool ok;
+      {
+        KJac J;
+        kin_jac(p, L.fT, x, J);
+        ok = ros_wsolve<LPC>(R, J, l, K, act, gh, dirmask, L.phi, f, k1, R.top[0]);
+        gC = kjac_sinkC(J, k1);
+        gN = kjac_sinkN(J, k1);
+      }
+      bool fail = group_any<LPC>(!ok);
+      if (!fail) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+          const double y = x[n] + h * k1[n];
+          f[n] = y < 0.0 ? 0.0 : y;
+        }
+        double c1[NV];
+#pragma unroll
+        for (int n = 0; n < NV; ++n) c1[n] = f[n];
+        column_rhs<LPC, ARITH>(p, L, bc, c1, s1C, s1N, [&](int n, double rr, double Fv) {
+          f[n] = rr - 2.0 * k1[n];
+          if (l == 0) F[1][n] = Fv;
+        });
+        KJac J;
+        kin_jac(p, L.fT, x, J);
+        ok = ros_wsolve<LPC>(R, J, l, K, act, gh, dirmask, L.phi, f, k2, R.top[1]);
+        fail = group_any<LPC>(!ok);
+        gC = gC + kjac_sinkC(J, k2);
+        gN = gN + kjac_sinkN(J, k2);
+      }
+      double bn = 0.0, bd = 1.0;
+      bool bad = false, neg = false;
+      if (!fail) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+          const double y = x[n] + h * (1.5 * k1[n] + 0.5 * k2[n]);
+          const double yp = y < 0.0 ? 0.0 : y;
+          const double sc = p.emp_atol + p.emp_rtol * (yp > x[n] ? yp : x[n]);
+          const double dv = fabs(h * (0.5 * k1[n] + 0.5 * k2[n]));
+          bad |= (dv != dv) || (sc != sc);
+          neg |= y < 0.0 - negtol;
+          if (dv * bd > bn * sc) {
+            bn = dv;
+            bd = sc;
+          }
+        }
+      }
+      double el = bn / bd;
+      bad |= el != el;
+      if (!act) {
+        el = 0.0;
+        bad = false;
+        neg = false;
+      }
+      const double err = lg_max<LPC>(el);
+      neg = group_any<LPC>(neg);
+      if (group_any<LPC>(bad) || (fail && !(h > p.emp_dt_min))) {
+        dg.status = 1;
+        if (l == 0) atomicOr(errflag, 1);
+        finish = true;
+      } else {
+        const bool accept = !fail && ((err <= 1.0 && !neg) || !(h > p.emp_dt_min));
+        double fac = err > 0.0 ? 0.9 * drsqrt(err) : 5.0;
+        if (fac < 0.2) fac = 0.2;
+        if (fac > 5.0) fac = 5.0;
+        if (accept) {
+#pragma unroll
+          for (int n = 0; n < NV; ++n) {
+            double y = x[n] + h * (1.5 * k1[n] + 0.5 * k2[n]);
+            if (y < 0.0) {
+              if (elem(n) >= 0 && act)
+                kadd(d.cellacc[(2 + elem(n)) * d.ncell + cell], d.cellacc[(7 + elem(n)) * d.ncell + cell], 0.0 - y);
+              y = 0.0;
+            }
+            x[n] = y;
+          }
+          __builtin_amdgcn_wave_barrier();
+          if (own) {
+            kadd(fint, fintc, (h * 0.5) * ((F[0][l] + F[1][l]) + gh * (R.top[0][l] + R.top[1][l])));
+            flast = F[1][l];
+          }
+          __builtin_amdgcn_wave_barrier();
+          kadd(sinkC, sinkCc, (h * 0.5) * ((s0C + s1C) + gh * gC));
+          kadd(sinkN, sinkNc, (h * 0.5) * ((s0N + s1N) + gh * gN));
+          rem = rem - h;
+          dg.accepted += 1;
+          if (err > 1.0 || neg) dg.forced += 1;
+          dg.dt_last = h;
+          if (h < dg.dt_minacc) dg.dt_minacc = h;
+          const double prop = h * fac;
+          dt = (clipped && dt > prop) ? dt : prop;
+        } else {
+          dg.rejected += 1;
+          if (fail) {
+            dg.newton += 1;
+            dt = h * 0.25;
+          } else if (err <= 1.0) {
+            dg.zero_sink += 1;
+            dt = h * 0.5;
+          } else {
+            dt = h * fac;
+          }
+        }
+        if (dt < p.emp_dt_min) dt = p.emp_dt_min;
+        if (dt > p.emp_dt_max) dt = p.emp_dt_max;
+      }
+    }
+    if (finish || pause) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n)
+        if (act) d.conc[n * d.ncell + cell] = x[n];
+      if (act) {
+        d.cellacc[cell] = sinkC;
+        d.cellacc[d.ncell + cell] = sinkN;
+        d.cellacc[5 * d.ncell + cell] = sinkCc;
+        d.cellacc[6 * d.ncell + cell] = sinkNc;
+      }
+      if (own) {
+        d.colacc[(long long)l * d.ncol + c] = fint;
+        d.colacc[(long long)(NV + l) * d.ncol + c] = fintc;
+        d.flux[(long long)l * d.ncol + c] = flast;
+      }
+      if (l == 0) {
+        dg.dt_next = dt;
+        d.diag[c] = dg;
+        work[c] = attempts < 0x7fffffffll ? (int)attempts : 0x7fffffff;
+        d.wrec[c] = pause ? r : r1 + 1;
+        d.wrem[c] = rem;
+        d.watt[c] = attempts;
+        if (pause) atomicAdd(pending, 1);
+      }
+      have = false;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // ADAPTIVE_EULER (solver_library.F90:104-140): the reject rule is global, so
 // each sub-step is a right-hand side, a domain-wide check and a commit
 // ---------------------------------------------------------------------------
@@ -897,8 +1326,9 @@ void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *
 }
 
 // blocks of a work-queue kernel that are resident at once on the current
-// device, cached per device (threads that race on a first query store the
-// same value)
+// device, cached per device and kernel instance (threads that race on a
+// first query store the same value).  PW = 2 selects the linearly implicit
+// kernel k_ros_adaptive, otherwise k_emp_adaptive with Patankar weights PW.
 template <int LPC, int AR, int BLK, int PW>
 static int emp_resident()
 {
@@ -912,8 +1342,12 @@ static int emp_resident()
   hipDeviceProp_t prop;
   int per_cu = 0;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive<LPC, AR, BLK, PW>, BLK, 0) != hipSuccess)
-    per_cu = 1;
+  hipError_t e;
+  if constexpr (PW == 2)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ros_adaptive<LPC, AR, BLK>, BLK, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive<LPC, AR, BLK, PW>, BLK, 0);
+  if (e != hipSuccess) per_cu = 1;
   v = (per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount;
   if (slot) slot->store(v, std::memory_order_relaxed);
   return v;
@@ -937,12 +1371,26 @@ static bool emp_shape_ok(int lpc, int blk)
   return (lpc == 16 || lpc == 32 || lpc == 64) && (blk == 64 || blk == 128 || blk == 256);
 }
 
+// the linearly implicit kernel has the reference and fast forms only
+#define SC_DISPATCH_ROS_ARITH(p, ...)                                          \
+  switch (arith_form(p)) {                                                     \
+    case ARITH_FAST: { constexpr int AR = ARITH_FAST; __VA_ARGS__; } break;     \
+    case ARITH_REF: { constexpr int AR = ARITH_REF; __VA_ARGS__; } break;       \
+    default: return false;                                                     \
+  }
+
+static bool ros_selected(const sedcol_params &p) { return p.adaptive_method == SEDCOL_ROS2_ADAPTIVE; }
+
 int emp_resident_blocks(int lpc, int blk, const sedcol_params &p)
 {
   if (!emp_shape_ok(lpc, blk)) return 0;
   int out = 0;
   auto body = [&]() -> bool {
-    SC_DISPATCH_PW(p, SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, out = emp_resident<LPC, AR, BLK, PW>()))));
+    if (ros_selected(p)) {
+      SC_DISPATCH_ROS_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, out = emp_resident<LPC, AR, BLK, 2>())));
+    } else {
+      SC_DISPATCH_PW(p, SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, out = emp_resident<LPC, AR, BLK, PW>()))));
+    }
     return true;
   };
   body();
@@ -954,6 +1402,16 @@ bool launch_emp_adaptive(int lpc, int blk, hipStream_t st, const Dev *d_dev, con
                          int *pending, int *work, int *errflag)
 {
   if (!emp_shape_ok(lpc, blk)) return false;
+  if (ros_selected(p)) {
+    SC_DISPATCH_ROS_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, {
+      const int G = BLK / LPC;
+      const unsigned need = (unsigned)((n + G - 1) / G);
+      const unsigned cap = (unsigned)emp_resident<LPC, AR, BLK, 2>();
+      hipLaunchKernelGGL((k_ros_adaptive<LPC, AR, BLK>), dim3(need < cap ? need : cap), dim3(BLK), 0, st, d_dev,
+                         list, n, d_params, d_ctl, next, pending, work, errflag);
+    })));
+    return true;
+  }
   SC_DISPATCH_PW(p, SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, {
     const int G = BLK / LPC;
     const unsigned need = (unsigned)((n + G - 1) / G);

@@ -25,6 +25,19 @@ Newton iteration (per cell):       70   (11 x (fma 2 + 3 mul + 1 add) + 4 for
                                          one (P, D) tree combine)
 RK4 step (per cell): 4 rhs + stage updates 3 x 11 x 2 + combination
   11 x (2 + 2) + sinks 12  = 692 + 66 + 44 + 12 = 814.
+Linearly implicit ROS2 attempt (method 7, per cell):
+  2 rhs                           346
+  reaction Jacobian kin_jac, x 2  254   (127 each: detritus 41, partition 59,
+                                         nitrification/ODU/P 27)
+  transport factor, 11 species    165   (15 per species and layer)
+  2 W-solves                     1578   (789 each: 2 species sweeps 2 x 132,
+                                         2 block solves 2 x 174, J_R k0 67,
+                                         transport residual 99, sum 11)
+  stage input and f1 - 2 k1        44
+  sink gradients                   24
+  error norm 11 x 13 + 1          144
+  accepted update + sinks          65
+  total                          2620
 """
 from __future__ import annotations
 
@@ -42,6 +55,13 @@ NEWTON_PER_CELL = NV * 6 + 4
 #: (13-term series, ~30 operations), the derivative ratio and two sums
 NEWTON_GEO_PER_CELL = NV * 7 + 36
 RK4_STEP_PER_CELL = 4 * RHS_PER_CELL + 3 * NV * 2 + NV * 4 + 12
+KIN_JAC = 41 + 59 + 27
+KJAC_SOLVE = 174
+KJAC_APPLY = 67
+SPECIES_SWEEP = 12 * NV
+W_SOLVE = 2 * SPECIES_SWEEP + 2 * KJAC_SOLVE + KJAC_APPLY + 9 * NV + NV
+ROS2_ATTEMPT_PER_CELL = (2 * RHS_PER_CELL + 2 * KIN_JAC + 15 * NV + 2 * W_SOLVE + 4 * NV + 24
+                         + (13 * NV + 1) + (5 * NV + 10))  # 2620
 
 #: HBM bytes a persistent adaptive launch moves per cell and per column,
 #: independent of the number of steps inside it: state in+out (2*11*8), the
@@ -62,6 +82,12 @@ def emp_flops(cells_per_col, attempts, newton, weights=0):
     per_newton = NEWTON_GEO_PER_CELL if weights == 1 else NEWTON_PER_CELL
     return float(np.sum(K * (EMP_ATTEMPT_PER_CELL * np.asarray(attempts, dtype=np.float64)
                              + per_newton * np.asarray(newton, dtype=np.float64))))
+
+
+def ros2_flops(cells_per_col, attempts):
+    """FLOPs of linearly implicit (method 7) work: attempts per column."""
+    K = np.asarray(cells_per_col, dtype=np.float64)
+    return float(np.sum(K * ROS2_ATTEMPT_PER_CELL * np.asarray(attempts, dtype=np.float64)))
 
 
 def rk4_flops(cells_per_col, steps):

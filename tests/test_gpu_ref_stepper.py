@@ -32,7 +32,9 @@ def _write_input(path, dom, x, p, method, nsteps, dt):
     with open(path, "wb") as f:
         f.write(np.array([ncol, K, method, nsteps], dtype=np.int32).tobytes())
         f.write(np.array([dt], dtype=np.float64).tobytes())
-        f.write(bytes(fill_struct(capi.Params, p)))
+        # the components only: Fortran stream I/O of a derived type reads no padding
+        last = capi.Params._fields_[-1][0]
+        f.write(bytes(fill_struct(capi.Params, p))[: getattr(capi.Params, last).offset + 4])
         for a in (dom.dz, dom.dzc, dom.phi, dom.phiI, dom.bfac):
             f.write(cellf(a).tobytes())
         t, c, u = dom.record(0)
