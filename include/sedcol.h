@@ -349,6 +349,16 @@ int sedcol_set_flux_cap(sedcol_handle *h, const double *flux_cap);
 int sedcol_assemble_forcing(sedcol_handle *h, const double *temp_surface, const double *c_surface,
                             const double *w_surface, const int32_t *present, double *flux_out);
 
+/* Per-cell release rates of the resident state under the current forcing
+ * (cell layout as sedcol_set_geometry; mmol m-3 pore water s-1): DIC from
+ * mineralisation (Cprod) and N2-N from denitrification (0.8 Denit), the
+ * counterpart of the diagnostic variables the reference reads from FABM
+ * after get_rhs (fabm_get_bulk_diagnostic_data, fabm_sediment_driver.F90:
+ * 562; hzg_omexdia_p_denit in the output of examples/standalone/omexdia_p/
+ * main.F90:109-157).  Land cells are 0.  SEDCOL_EINVAL while a forcing
+ * series is active. */
+int sedcol_rates(sedcol_handle *h, double *dic_release, double *n2_release);
+
 /* --- restart ------------------------------------------------------------ */
 
 /* Checkpoint of the device-resident integration state: state, budget

@@ -944,6 +944,38 @@ int sedcol_rhs_noalloc(sedcol_handle *h, const double *conc, double *rhs, double
   return SEDCOL_OK;
 }
 
+int sedcol_rates(sedcol_handle *h, double *dic_release, double *n2_release)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  const DeviceScope dscope(h->device);
+  if (!dic_release || !n2_release) return fail(SEDCOL_EINVAL, "null argument");
+  if (h->nrec > 1) return fail(SEDCOL_EINVAL, "a forcing series is active; the rates use one record (sedcol_set_forcing)");
+  if ((rc = ensure_scratch(h))) return rc;
+  const size_t nc = (size_t)h->ncell;
+  HIPOK(hipMemsetAsync(h->d_scr, 0, sizeof(double) * 2 * nc, h->stream));
+  const Dev d = h->dev();
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty())
+      sc::launch_rhs(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), h->p, 0, h->d_conc, nullptr,
+                     nullptr, h->d_scr);
+  HIPOK(hipGetLastError());
+  std::vector<double> buf(2 * nc);
+  HIPOK(hipMemcpyAsync(buf.data(), h->d_scr, sizeof(double) * buf.size(), hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  if (h->layout == SEDCOL_LAYOUT_PACKED) {
+    std::memcpy(dic_release, buf.data(), sizeof(double) * nc);
+    std::memcpy(n2_release, buf.data() + nc, sizeof(double) * nc);
+  } else {
+    for (int c = 0; c < h->ncol; ++c)
+      for (int k = 0; k < h->kmax; ++k) {
+        dic_release[fidx(h, c, k)] = buf[(size_t)c * h->kmax + k];
+        n2_release[fidx(h, c, k)] = buf[nc + (size_t)c * h->kmax + k];
+      }
+  }
+  return SEDCOL_OK;
+}
+
 // ADAPTIVE_EULER over dt on the resident state (solver_library.F90:104-140)
 static int adaptive_euler(sedcol_handle *h, double dt, int rec, int *launches)
 {

@@ -71,7 +71,7 @@ __device__ __forceinline__ bool group_any(bool b)
 
 template <int LPC, int ARITH>
 __global__ __launch_bounds__(BLOCK) void k_rhs(Dev d, const int *list, int nlist, sedcol_params p, int rec,
-                                               const double *conc, double *rhs, double *flux)
+                                               const double *conc, double *rhs, double *flux, double *sinks)
 {
   constexpr int G = BLOCK / LPC;
   __shared__ double sbc[G][2 * NV];
@@ -93,9 +93,13 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(Dev d, const int *list, int nlist
   for (int n = 0; n < NV; ++n) x[n] = L.active ? conc[n * d.ncell + cell] : 0.0;
   double sC, sN;
   column_rhs<LPC, ARITH>(p, L, sbc[g], x, sC, sN, [&](int n, double r, double F) {
-    if (L.active) rhs[n * d.ncell + cell] = r;
+    if (L.active && rhs) rhs[n * d.ncell + cell] = r;
     if (l == 0 && flux) flux[(long long)n * d.ncol + c] = F;
   }, &d.zw);
+  if (L.active && sinks) {
+    sinks[cell] = sC;
+    sinks[d.ncell + cell] = sN;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1289,10 +1293,10 @@ static inline int arith_form(const sedcol_params &p)
   }
 
 void launch_rhs(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
-                const double *conc, double *rhs, double *flux)
+                const double *conc, double *rhs, double *flux, double *sinks)
 {
   SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_rhs<LPC, AR>), grid_for<LPC>(n), dim3(BLOCK), 0,
-                                                               st, d, list, n, p, rec, conc, rhs, flux)));
+                                                               st, d, list, n, p, rec, conc, rhs, flux, sinks)));
 }
 
 void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *list, int n,

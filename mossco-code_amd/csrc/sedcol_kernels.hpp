@@ -43,8 +43,11 @@ struct EmpCtl {
   int resume;   // 1: continue every column from d.wrec / d.wrem / d.watt
 };
 
+// rhs (NV*ncell), top fluxes (NV*ncol) and the per-cell DIC and N2 release
+// rates (2*ncell: sinks[cell], sinks[ncell + cell]) of conc; any output may
+// be null
 void launch_rhs(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const sedcol_params &p, int rec,
-                const double *conc, double *rhs, double *flux);
+                const double *conc, double *rhs, double *flux, double *sinks = nullptr);
 void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *list, int n,
                   const sedcol_params &p, int rec, double dt, int nsteps, int *errflag);
 // Adaptive EMP over the span in *d_ctl for the n columns of list, lpc lanes

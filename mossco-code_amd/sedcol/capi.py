@@ -34,7 +34,7 @@ NBUD = 16
 EXPORTS = (
     "sedcol_create", "sedcol_destroy", "sedcol_last_error", "sedcol_set_params", "sedcol_set_geometry",
     "sedcol_set_forcing", "sedcol_set_forcing_series", "sedcol_set_state", "sedcol_get_state", "sedcol_rhs",
-    "sedcol_rhs_noalloc",
+    "sedcol_rhs_noalloc", "sedcol_rates",
     "sedcol_step", "sedcol_run", "sedcol_integrate", "sedcol_get_fluxes", "sedcol_get_budget",
     "sedcol_reset_budget", "sedcol_get_inventory", "sedcol_get_diagnostics", "sedcol_reset_diagnostics",
     "sedcol_residual", "sedcol_get_last_min_dt", "sedcol_synchronize", "sedcol_last_timing",
@@ -121,6 +121,7 @@ def lib() -> ctypes.CDLL:
             "sedcol_get_state": (ctypes.c_int, [vp, pd]),
             "sedcol_rhs": (ctypes.c_int, [vp, pd, pd, pd]),
             "sedcol_rhs_noalloc": (ctypes.c_int, [vp, pd, pd, pd]),
+            "sedcol_rates": (ctypes.c_int, [vp, pd, pd]),
             "sedcol_step": (ctypes.c_int, [vp, dbl, i32, pd, pd]),
             "sedcol_run": (ctypes.c_int, [vp, dbl, i32, i32, i32]),
             "sedcol_integrate": (ctypes.c_int, [vp, dbl, dbl]),
@@ -285,6 +286,15 @@ class SedimentColumns:
         f = np.empty((NVAR, self.ncol))
         check(lib().sedcol_rhs(self._h, _pd(x), _pd(r), _pd(f)))
         return r, f
+
+    def rates(self):
+        """(DIC release, N2-N release) per cell of the resident state
+        (mmol m-3 pore water s-1), cell layout of the handle."""
+        shape = (self.ncell,) if self.layout == LAYOUT_PACKED else (self.kmax, self.ncol)
+        a = np.empty(shape)
+        b = np.empty(shape)
+        check(lib().sedcol_rates(self._h, _pd(a), _pd(b)))
+        return a, b
 
     def get_rhs_noalloc(self, conc):
         """get_rhs through sedcol_rhs_noalloc (no allocation inside the call)."""

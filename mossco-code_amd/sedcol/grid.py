@@ -6,7 +6,8 @@ domain on the host and whose results the library receives through
 
 * ``init_grid``           src/drivers/fabm_sediment_driver.F90:127-177
 * porosity / bioturbation in ``initialize`` :278-303
-* ``update_porosity``     :393-442 (interface porosity :434-435)
+* ``update_porosity``     :393-442 (from the surface value :408-428, interface
+  porosity :434-435)
 
 plus the geometric grid, exponential porosity and "constant then
 exponential" bioturbation profile that the benchmark configurations use.
@@ -107,6 +108,40 @@ def bioturbation_const_exp(zi_upper, mixed_depth=0.05, length=0.05):
     """1 down to mixed_depth, then exp(-(z - mixed_depth)/length)."""
     zi = np.asarray(zi_upper, dtype=np.float64)
     return np.where(zi <= mixed_depth, 1.0, np.exp(-(zi - mixed_depth) / length))
+
+
+def update_porosity(phi_surface, zc, dz, off, porosity_fac=0.9, pom_flux_max=2.0e4, mask=None):
+    """update_porosity(from_surface=.true.) of fabm_sediment_driver.F90:
+    393-442 on packed columns (column c owns cells off[c]..off[c+1]-1).
+    The caller's porosity import sets the top layer (fabm_sediment_component.
+    F90:1642-1643); then per column
+      phi_k = phi_1 (1 - porosity_fac (zc_k - zc_1)),  k >= 2   (:410-413)
+      flux_cap_k = pom_flux_max/86400 (1 - phi_k) dz_k, not increasing
+      below the second layer                                    (:415-426)
+      phi = 1 on masked columns                                 (:431)
+      interface porosity phiI_1 = phi_1, phiI_k = (phi_k-1 + phi_k)/2 (:434-435).
+    phi_surface is per column; returns (phi, phiI, flux_cap) per cell, to be
+    handed to sedcol_set_geometry / sedcol_set_flux_cap."""
+    zc = np.asarray(zc, dtype=np.float64)
+    dz = np.asarray(dz, dtype=np.float64)
+    off = np.asarray(off, dtype=np.int64)
+    ncol = off.size - 1
+    phi_surface = np.broadcast_to(np.asarray(phi_surface, dtype=np.float64), (ncol,))
+    phi = np.empty_like(zc)
+    phiI = np.empty_like(zc)
+    fc = np.empty_like(zc)
+    for c in range(ncol):
+        sl = slice(off[c], off[c + 1])
+        z = zc[sl]
+        p = np.empty(z.size)
+        p[0] = phi_surface[c]
+        p[1:] = phi_surface[c] * (1.0 - porosity_fac * (z[1:] - z[0]))
+        fc[sl] = flux_cap(p, dz[sl], pom_flux_max)
+        if mask is not None and mask[c]:
+            p[:] = 1.0
+        phi[sl] = p
+        phiI[sl] = interface_porosity(p)
+    return phi, phiI, fc
 
 
 def flux_cap(phi, dz, pom_flux_max=2.0e4):
