@@ -157,6 +157,36 @@ def deposition(s: dict, frac_labile=0.7) -> np.ndarray:
     return fup
 
 
+#: domains of at least this many columns build their geometry with array
+#: operations (the same formulas; numpy's power may round differently in the
+#: last bit from the per-column path)
+LARGE = 20000
+
+
+def _geometric_cells(nlayers, total_depth, dz1, off, colid):
+    """Cell geometry of geometric columns, vectorised over all cells."""
+    r = grid.geometric_ratios(nlayers, dz1, total_depth)
+    first = off[:-1]
+    kidx = np.arange(colid.size) - np.repeat(first, nlayers)
+    dz = dz1 * r[colid] ** kidx
+    # interface depths: running sums of dz within each column, top first
+    # (columns of one layer count at a time, as (n, K) rows)
+    zi = np.empty_like(dz)
+    for K in np.unique(nlayers):
+        cols = np.flatnonzero(nlayers == K)
+        cells = (first[cols][:, None] + np.arange(K)[None, :])
+        d = dz[cells]
+        z = np.zeros_like(d)
+        if K > 1:
+            z[:, 1:] = np.cumsum(d[:, :-1], axis=1)
+        zi[cells] = z
+    zc = zi + 0.5 * dz
+    dzc = np.empty_like(dz)
+    dzc[1:] = zc[1:] - zc[:-1]
+    dzc[first] = 0.0
+    return dz, dzc, zc, zi
+
+
 def build_columns(nlayers, total_depth, s: dict, dz1=1.0e-3, mask=None, params=None, meta=None) -> Domain:
     """Geometry and initial state of columns with geometric layers."""
     nlayers = np.asarray(nlayers, dtype=np.int32)
@@ -165,23 +195,26 @@ def build_columns(nlayers, total_depth, s: dict, dz1=1.0e-3, mask=None, params=N
     off = np.zeros(ncol + 1, dtype=np.int64)
     np.cumsum(nlayers, out=off[1:])
     ncell = int(off[-1])
-    dz = np.empty(ncell)
-    dzc = np.empty(ncell)
-    zc = np.empty(ncell)
-    zi = np.empty(ncell)
-    cache = {}
-    for c in range(ncol):
-        key = (int(nlayers[c]), float(total_depth[c]))
-        g = cache.get(key)
-        if g is None:
-            g = grid.init_grid_geometric(key[0], dz1, key[1])
-            cache[key] = g
-        sl = slice(off[c], off[c + 1])
-        dz[sl] = g["dz"]
-        dzc[sl] = g["dzc"]
-        zc[sl] = g["zc"]
-        zi[sl] = g["zi"][:-1]
     colid = np.repeat(np.arange(ncol), nlayers)
+    if ncol >= LARGE:
+        dz, dzc, zc, zi = _geometric_cells(nlayers, total_depth, dz1, off, colid)
+    else:
+        dz = np.empty(ncell)
+        dzc = np.empty(ncell)
+        zc = np.empty(ncell)
+        zi = np.empty(ncell)
+        cache = {}
+        for c in range(ncol):
+            key = (int(nlayers[c]), float(total_depth[c]))
+            g = cache.get(key)
+            if g is None:
+                g = grid.init_grid_geometric(key[0], dz1, key[1])
+                cache[key] = g
+            sl = slice(off[c], off[c + 1])
+            dz[sl] = g["dz"]
+            dzc[sl] = g["dzc"]
+            zc[sl] = g["zc"]
+            zi[sl] = g["zi"][:-1]
     phi = grid.porosity_exponential(zc, s["phi0"][colid], s["phi_inf"][colid])
     phiI = np.empty(ncell)
     first = off[:-1]

@@ -51,6 +51,48 @@ def geometric_ratio(K: int, dz1: float, total: float) -> float:
     return 0.5 * (lo + hi)
 
 
+def geometric_ratios(K, dz1: float, total):
+    """geometric_ratio for arrays of layer counts and totals: the same
+    bisection, element by element (each element stops at the step the
+    scalar version stops at), so the ratios are the same numbers."""
+    K = np.asarray(K, dtype=np.int64)
+    total = np.asarray(total, dtype=np.float64)
+    K, total = np.broadcast_arrays(K, total)
+    out = np.ones(K.shape)
+    uniform = np.abs(K * dz1 - total) <= 1e-15 * total
+    todo = ~uniform
+    if not todo.any():
+        return out
+    Kf, T = K[todo].astype(np.float64), total[todo]
+    Kint = K[todo]
+
+    def s(r):
+        one = r == 1.0
+        v = np.where(one, dz1 * Kf, 0.0)
+        rr = np.where(one, 2.0, r)
+        return np.where(one, v, dz1 * (rr ** Kint - 1.0) / (rr - 1.0))
+
+    grow = Kint * dz1 < T
+    lo = np.where(grow, 1.0, 1e-6)
+    hi = np.where(grow, 2.0, 1.0)
+    while True:
+        short = s(hi) < T
+        if not short.any():
+            break
+        hi = np.where(short, hi * 2.0, hi)
+    live = np.ones(lo.shape, dtype=bool)
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        below = s(mid) < T
+        lo = np.where(live & below, mid, lo)
+        hi = np.where(live & ~below, mid, hi)
+        live &= ~(hi - lo <= 1e-16 * hi)
+        if not live.any():
+            break
+    out[todo] = 0.5 * (lo + hi)
+    return out
+
+
 def init_grid_geometric(K: int, dz1: float, total: float):
     """Geometric layers: top layer dz1, constant growth ratio, given total."""
     r = geometric_ratio(K, dz1, total)

@@ -122,16 +122,55 @@ def layer_centres(nlayers: int, depth: float, dz1: float = 1.0e-3) -> np.ndarray
     return grid.init_grid_geometric(int(nlayers), dz1, float(depth))["zc"]
 
 
+def _centres_many(nlayers: int, depth: np.ndarray, dz1: float = 1.0e-3) -> np.ndarray:
+    """(n, K) layer centres of geometric columns of the given depths."""
+    r = grid.geometric_ratios(np.full(depth.size, nlayers), dz1, depth)
+    dz = dz1 * r[:, None] ** np.arange(nlayers)[None, :]
+    zi = np.zeros_like(dz)
+    zi[:, 1:] = np.cumsum(dz[:, :-1], axis=1)
+    return zi + 0.5 * dz
+
+
+def _interp_rows(zk: np.ndarray, z30: np.ndarray, f: np.ndarray) -> np.ndarray:
+    """Row-wise linear interpolation, constant beyond the end points:
+    zk (n, K) targets, z30 (n, M) increasing nodes, f (n, V, M) -> (n, V, K)."""
+    n, M = z30.shape
+    j = np.empty(zk.shape, dtype=np.int64)
+    for i0 in range(0, n, 65536):  # bounded temporaries
+        i1 = min(n, i0 + 65536)
+        j[i0:i1] = (zk[i0:i1, :, None] >= z30[i0:i1, None, :]).sum(axis=2) - 1
+    j = np.clip(j, 0, M - 2)
+    rows = np.arange(n)[:, None]
+    z0, z1 = z30[rows, j], z30[rows, j + 1]
+    w = np.clip((zk - z0) / (z1 - z0), 0.0, 1.0)
+    f0 = np.take_along_axis(f, j[:, None, :], axis=2)
+    f1 = np.take_along_axis(f, j[:, None, :] + 1, axis=2)
+    return f0 + w[:, None, :] * (f1 - f0)
+
+
 def from_targets(y: np.ndarray, depth, nlayers) -> np.ndarray:
     """(n, NV*30) log10 profiles on each column's 30-layer grid -> packed
     state (NV, sum K) on K-layer grids of the same depth: linear
     interpolation of the log10 profile in depth (constant beyond the first
-    and last centres), then 10**y - eps clipped at 0."""
+    and last centres), then 10**y - eps clipped at 0.  Large batches are
+    interpolated with array operations, one layer count at a time."""
     y = np.asarray(y, dtype=np.float64)
     n = y.shape[0]
     depth = np.broadcast_to(np.asarray(depth, dtype=np.float64), (n,))
     nlayers = np.broadcast_to(np.asarray(nlayers, dtype=np.int64), (n,))
     prof = y.reshape(n, NVAR, K_SURR)
+    if n >= 2048:
+        off = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(nlayers, out=off[1:])
+        out = np.empty((NVAR, int(off[-1])))
+        z30 = _centres_many(K_SURR, np.ascontiguousarray(depth))
+        for K in np.unique(nlayers):
+            cols = np.flatnonzero(nlayers == K)
+            zk = _centres_many(int(K), np.ascontiguousarray(depth[cols]))
+            v = _interp_rows(zk, z30[cols], prof[cols])  # (nc, NV, K)
+            cells = off[cols][:, None] + np.arange(K)[None, :]
+            out[:, cells.ravel()] = v.transpose(1, 0, 2).reshape(NVAR, -1)
+        return np.maximum(10.0 ** out - EPS[:, None], 0.0)
     out = np.empty((NVAR, int(nlayers.sum())))
     o = 0
     cache = {}

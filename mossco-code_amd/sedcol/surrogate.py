@@ -281,23 +281,27 @@ def predict(model: Model, X, net: Network | None = None):
 
 
 def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_tol=1.0e-6, flux_rtol=0.01,
-             tails=0.05, emp_weights=0, log=None):
-    """Surrogate warm start against cold spin-up on n drawn columns (layers
-    U{20..40}, a fraction `tails` of the inputs outside the training ranges).
+             tails=0.05, adaptive_method=7, dt_max=86400.0, chunk_days=1.0, log=None):
+    """Surrogate warm start against plain cold spin-up to steady state on n
+    drawn columns (layers U{20..40}, a fraction `tails` of the inputs outside
+    the training ranges).  Every integration uses the same stepper
+    (adaptive_method, steps up to dt_max) and retires a column at the first
+    chunk boundary where its steady-state residual is below res_tol.
 
-    cold: the cold spin-up the reference performs (presimulation of `years`
-          from the cold state, fabm_sediment_component.F90:557-632), per
-          column; its final top fluxes F_cold are the reference.
-    warm: surrogate prediction interpolated to each column's layers, then
-          the plain integration, each column retired at the first day on
-          which its residual is below res_tol and its top fluxes are within
-          flux_rtol of F_cold; columns not retired within warm_days fall back
-          to the cold spin-up.
-    The wall times (warm including prediction and fallback) and the
-    integration work (attempts x layers) of both are reported, with the
-    warm start's errors per variable.  emp_weights selects the Patankar
-    factor of every integration (0 product, 1 geometric mean).  Returns a
-    dict."""
+    1. cold spin-up of all columns from the cold state, capped at `years`
+       (the reference's presimulation, fabm_sediment_component.F90:557-632,
+       per column).  Columns that reach the residual criterion form the set
+       S; their top fluxes at that point are F_cold.  The others never
+       become stationary within the cap and are reported, not compared.
+    2. cold: the cold spin-up of S alone, timed (every column of S retires).
+    3. warm: prediction, interpolation to each column's layers, then the
+       integration of S, a column retiring once its residual is below
+       res_tol and every top flux is within flux_rtol of F_cold (floor: 1 %
+       of the carbon deposition); columns not retired within warm_days
+       fall back to a cold start (their cold spin-up, timed and added).
+    Returns wall times of each phase, the ratio cold / warm, the fallback
+    rate, and per-variable errors of the warm start and of the retired
+    warm states' fluxes."""
     import time
 
     from . import capi, spinup
@@ -306,42 +310,46 @@ def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_
     s = spinup.draw_inputs(rng, n, tails=tails)
     K = rng.integers(20, 41, size=n)
     dom = spinup.build(s, K)
-    dom.params.emp_weights = emp_weights
-    floor = spinup.flux_floor(s)
+    dom.params.adaptive_method = adaptive_method
+    dom.params.emp_dt_max = dt_max
     say = log or (lambda m: None)
 
-    say(f"cold: spin-up of {n} columns for {years} y")
-    cold = spinup.spin(dom, years, 1e-300, chunk_days=10.0, log=say)
-    F_cold = cold["flux"]
+    say(f"1. cold spin-up of {n} columns, cap {years} y")
+    full = spinup.spin(dom, years, res_tol, chunk_days=chunk_days, log=say)
+    conv = full["t_conv"] >= 0.0
+    S = np.flatnonzero(conv)
+    sub = dom.subset(S)
+    ss = {k: (np.asarray(v)[S] if np.ndim(v) else v) for k, v in s.items()}
+    F_cold = full["flux"][:, S]
+    floor = spinup.flux_floor(ss)
+    say(f"2. cold spin-up of the {S.size} columns that become stationary")
+    cold = spinup.spin(sub, years, res_tol, chunk_days=chunk_days, log=say)
+    assert np.all(cold["t_conv"] >= 0.0)
 
-    say("warm: prediction and short integration")
-    X = spinup.as_matrix(s)
+    say("3. warm start")
+    X = spinup.as_matrix(ss)
+    t0 = time.perf_counter()
     net = Network(model.shape, max_batch=4096)
-    g = capi.SedimentColumns.from_domain(dom)
     try:
         net.set_params(model.params)
-        t0 = time.perf_counter()
         yn = net.predict((X - model.x_mean) / model.x_std)
-        y = yn.astype(np.float64) * model.y_std + model.y_mean
-        x0 = spinup.from_targets(y, s["depth"], K)
-        t_pred = time.perf_counter() - t0
         pred_ms, _ = net.last_timing()
-        warm = spinup.spin(dom, warm_days / 365.0, res_tol, chunk_days=1.0, conc=x0, flux_ref=F_cold,
-                           flux_rtol=flux_rtol, floor=floor, columns=g, log=say)
     finally:
         net.close()
-        g.close()
+    y = yn.astype(np.float64) * model.y_std + model.y_mean
+    t_pred = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    x0 = spinup.from_targets(y, ss["depth"], K[S])
+    t_interp = time.perf_counter() - t0
+    warm = spinup.spin(sub, warm_days / 365.0, res_tol, chunk_days=chunk_days, conc=x0, flux_ref=F_cold,
+                       flux_rtol=flux_rtol, floor=floor, log=say)
     fb = warm["t_conv"] < 0.0
-    t_fb, att_fb = 0.0, 0.0
+    t_fb = 0.0
     if fb.any():
-        sub = dom.subset(np.flatnonzero(fb))
         say(f"fallback: cold spin-up of {int(fb.sum())} columns")
-        r = spinup.spin(sub, years, 1e-300, chunk_days=10.0, log=say)
+        r = spinup.spin(sub.subset(np.flatnonzero(fb)), years, res_tol, chunk_days=chunk_days, log=say)
         t_fb = r["wall_s"]
-        att_fb = float(np.sum(r["attempts"] * K[fb]))
-    wall_warm = t_pred + warm["wall_s"] + t_fb
-    work_cold = float(np.sum(cold["attempts"] * K))
-    work_warm = float(np.sum(warm["attempts"] * K)) + att_fb
+    wall_warm = t_pred + t_interp + warm["wall_s"] + t_fb
 
     lg = lambda a: np.log10(np.maximum(a, 0.0) + spinup.EPS[:, None])
     prof_err = np.abs(lg(x0) - lg(cold["state"]))
@@ -353,25 +361,28 @@ def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_
         per_var[name] = {
             "warm_start_log10_err_median": float(np.median(prof_err[v])),
             "warm_start_log10_err_p95": float(np.quantile(prof_err[v], 0.95)),
-            "flux_rel_err_median_accepted": float(np.median(fl_err[v][ok])) if ok.any() else None,
-            "flux_rel_err_median_all": float(np.median(fl_err[v])),
+            "flux_rel_err_median_retired": float(np.median(fl_err[v][ok])) if ok.any() else None,
+            "flux_rel_err_max_retired": float(np.max(fl_err[v][ok])) if ok.any() else None,
         }
-    tail = s["tail"]
+    tail = ss["tail"]
+    tc = cold["t_conv"]
     return {
-        "columns": n, "tail_fraction": float(tail.mean()), "years": years, "warm_days": warm_days,
-        "emp_weights": "geometric" if emp_weights else "product",
-        "res_tol": res_tol, "flux_rtol": flux_rtol,
+        "columns": n, "tail_fraction": float(np.mean(s["tail"])), "years_cap": years, "warm_days": warm_days,
+        "adaptive_method": adaptive_method, "dt_max_s": dt_max, "res_tol": res_tol, "flux_rtol": flux_rtol,
+        "stationary_columns": int(S.size), "non_stationary_columns": int(n - S.size),
+        "non_stationary_tail_columns": int(np.sum(s["tail"][~conv])),
         "wall_cold_s": cold["wall_s"], "wall_warm_s": wall_warm, "speedup_wall": cold["wall_s"] / wall_warm,
-        "wall_predict_s": t_pred, "predict_kernel_ms": pred_ms, "wall_warm_integration_s": warm["wall_s"],
-        "wall_fallback_s": t_fb,
-        "work_cold_cell_attempts": work_cold, "work_warm_cell_attempts": work_warm,
-        "speedup_work": work_cold / work_warm if work_warm > 0 else None,
+        "wall_predict_s": t_pred, "predict_kernel_ms": pred_ms, "wall_interpolate_s": t_interp,
+        "wall_warm_integration_s": warm["wall_s"], "wall_fallback_s": t_fb,
         "fallback_rate": float(fb.mean()),
-        "fallback_rate_in_range": float(fb[~tail].mean()) if (~tail).any() else None,
         "fallback_rate_tail": float(fb[tail].mean()) if tail.any() else None,
-        "warm_days_to_criterion_median": float(np.median(warm["t_conv"][ok]) / 86400.0) if ok.any() else None,
+        "cold_days_to_steady_median": float(np.median(tc) / 86400.0),
+        "cold_days_to_steady_max": float(np.max(tc) / 86400.0),
+        "warm_days_to_steady_median": float(np.median(warm["t_conv"][ok]) / 86400.0) if ok.any() else None,
+        "warm_days_to_steady_max": float(np.max(warm["t_conv"][ok]) / 86400.0) if ok.any() else None,
         "cold_attempts_per_column_median": float(np.median(cold["attempts"])),
         "cold_attempts_per_column_max": float(np.max(cold["attempts"])),
+        "warm_attempts_per_column_median": float(np.median(warm["attempts"])),
         "per_variable": per_var,
     }
 
@@ -402,7 +413,8 @@ def _main(argv=None):
     e.add_argument("--seed", type=int, default=7)
     e.add_argument("--years", type=float, default=2.0)
     e.add_argument("--warm-days", type=float, default=30.0)
-    e.add_argument("--emp-weights", type=int, default=0, help="Patankar factor: 0 product, 1 geometric mean")
+    e.add_argument("--method", type=int, default=7, help="adaptive stepper: 6 EMP, 7 linearly implicit ROS2")
+    e.add_argument("--dt-max", type=float, default=86400.0)
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "train":
@@ -439,7 +451,8 @@ def _main(argv=None):
                           "predict_tflops": flops_per_sample() * a.rows / (pms * 1e-3) / 1e12}), flush=True)
     else:
         model = Model.load(a.model)
-        r = evaluate(model, a.n, a.seed, years=a.years, warm_days=a.warm_days, emp_weights=a.emp_weights, log=log)
+        r = evaluate(model, a.n, a.seed, years=a.years, warm_days=a.warm_days, adaptive_method=a.method,
+                     dt_max=a.dt_max, log=log)
         print(json.dumps(r), flush=True)
 
 
