@@ -243,6 +243,44 @@ def ragged_bench(args):
     return 0 if same else 1
 
 
+def spinup_bench(args):
+    """Spin-up wall time, surrogate warm start against the plain cold
+    spin-up (the second half of the metric): sedcol.surrogate.evaluate on
+    --spin-columns drawn columns (layers U{20..40}, 5 % of the inputs outside
+    the training ranges), cold = the reference's presimulation of
+    --spin-years per column, warm = prediction + integration retiring each
+    column at residual < 1e-6 with every top flux within 1 % of the cold
+    result, fallback to the cold spin-up after 30 days; one stepper for all
+    (--spin-method, steps up to one day)."""
+    import torch
+
+    from sedcol import surrogate
+
+    torch.cuda.init()
+    model = surrogate.Model.load(REPO / "mossco-code_amd" / "sedcol" / "models" / "spinup_mlp.npz")
+    r = surrogate.evaluate(model, args.spin_columns, args.seed if args.seed else 7, years=args.spin_years,
+                           warm_days=30.0, adaptive_method=args.spin_method, dt_max=86400.0,
+                           log=lambda m: print(m, file=sys.stderr, flush=True))
+    out = {
+        "metric": "spin-up wall time to steady state, plain cold spin-up / surrogate warm start",
+        "value": r["speedup_wall"],
+        "unit": "x",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64 integration, bf16 network",
+        "data": "synthetic",
+        "config": {"workload": f"{args.spin_columns} columns, K U{{20..40}}, 5% inputs outside the training "
+                               f"ranges; cold = {args.spin_years} y presimulation per column; warm = BF16 MLP "
+                               "prediction + integration to residual < 1e-6 and fluxes within 1% of cold, "
+                               "fallback to cold after 30 d",
+                   "adaptive_method": args.spin_method, "seed": args.seed if args.seed else 7},
+        **{k: v for k, v in r.items() if k != "per_variable"},
+        "per_variable": r["per_variable"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,9 +293,15 @@ def main(argv=None):
     ap.add_argument("--arith", type=int, default=1, help="0 reference operation order, 1 fast form")
     ap.add_argument("--emp-weights", type=int, default=0,
                     help="Patankar factor of adaptive EMP: 0 product (default), 1 geometric mean")
-    ap.add_argument("--workload", choices=("shelf", "ragged", "rk4"), default="shelf",
+    ap.add_argument("--workload", choices=("shelf", "ragged", "rk4", "spinup"), default="shelf",
                     help="shelf: the benchmark line; ragged: schedule/autotuner comparison on ragged columns; "
-                         "rk4: the plain 4x4 RK4 case beside the reference's own stepper on the CPU")
+                         "rk4: the plain 4x4 RK4 case beside the reference's own stepper on the CPU; "
+                         "spinup: surrogate warm start against cold spin-up (wall-time ratio)")
+    ap.add_argument("--spin-columns", type=int, default=4000, help="spinup: columns")
+    ap.add_argument("--spin-years", type=float, default=2.0, help="spinup: cold presimulation (years)")
+    ap.add_argument("--spin-method", type=int, default=7, help="spinup: 6 adaptive EMP, 7 linearly implicit")
+    ap.add_argument("--adaptive-method", type=int, default=6,
+                    help="shelf: adaptive stepper, 6 EMP (the configuration's) or 7 linearly implicit")
     ap.add_argument("--ncol", type=int, default=100000, help="ragged: columns")
     ap.add_argument("--span-hours", type=float, default=24.0, help="ragged: timed span per schedule")
     ap.add_argument("--window", type=int, default=100, help="ragged: attempts per column per window")
@@ -274,6 +318,8 @@ def main(argv=None):
         return ragged_bench(args)
     if args.workload == "rk4":
         return rk4_bench(args)
+    if args.workload == "spinup":
+        return spinup_bench(args)
 
     import torch
 
@@ -285,6 +331,7 @@ def main(argv=None):
     dom = cases.shelf_sea(n=args.n, days=days, seed=args.seed)
     dom.params.arith = args.arith
     dom.params.emp_weights = args.emp_weights
+    dom.params.adaptive_method = args.adaptive_method
     gen_s = time.perf_counter() - t
     wet = dom.mask == 0
     K = dom.nlayers
@@ -315,7 +362,8 @@ def main(argv=None):
     newt = (d1["newton"] - d0["newton"])[wet]
     colsteps = int(acc.sum())
     value = colsteps / wall
-    fl = flops.emp_flops(K[wet], acc + rej, newt, weights=args.emp_weights)
+    ros = args.adaptive_method == 7
+    fl = flops.ros2_flops(K[wet], acc + rej) if ros else flops.emp_flops(K[wet], acc + rej, newt, weights=args.emp_weights)
     avg_launch_s = kernel_ms * 1e-3 / max(launches, 1)
     achieved = fl / max(launches, 1) / avg_launch_s / 1e12
     xs = g.get_state()
@@ -325,7 +373,8 @@ def main(argv=None):
     res = budget.closure(inv_start, g.inventory(), g.budget())
     closure_max = {e: float(r[wet].max()) for e, r in res.items()}
     g.close()
-    default_workload = args.n == 300 and args.seed == 0 and args.arith == 1 and args.emp_weights == 0
+    default_workload = (args.n == 300 and args.seed == 0 and args.arith == 1 and args.emp_weights == 0
+                        and not ros)
     traffic = DEFAULT_TRAFFIC_BYTES if default_workload else None
 
     out = {
@@ -351,13 +400,14 @@ def main(argv=None):
             "seed": args.seed,
             "arith": "fast" if args.arith else "reference",
             "emp_weights": "geometric" if args.emp_weights else "product",
+            "adaptive_method": "linearly implicit ROS2 (7)" if ros else "EMP (6)",
             "forcing_phase": f"days {args.warmup}-{args.warmup + args.steps - 1} of a seasonal cycle whose "
                              "coldest, most oxygenated bottom water is on day 0 (sin(2 pi d/365 - pi/2))",
         },
         "roofline": {
             "bound": "fp64-valu",
             "pipe": "FP64 VALU (MI355X FP64 vector peak 78.6 TFLOP/s; no matrix work on this path)",
-            "kernel": "k_emp_adaptive<32>",
+            "kernel": "k_ros_adaptive<32>" if ros else "k_emp_adaptive<32>",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
