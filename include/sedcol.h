@@ -292,7 +292,9 @@ int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t rep
  * presimulation plus broadcast of fabm_sediment_component.F90:557-632 (the
  * reference integrates one column for presimulation_years with a fixed dt of
  * 3600 s and copies it to every wet column).  Every wet column is integrated
- * by adaptive EMP from its resident state under the current forcing, in
+ * by the adaptive stepper that sedcol_params.adaptive_method selects
+ * (adaptive EMP, or the linearly implicit method 7, the faster choice near
+ * steady state) from its resident state under the current forcing, in
  * chunks of `chunk` seconds from t_start, until its steady-state residual
  * (sedcol_residual with cfloor) is below tol or t_max seconds have passed.
  * A column is retired from the integration as soon as it is converged.

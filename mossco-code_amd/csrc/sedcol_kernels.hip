@@ -623,50 +623,63 @@ struct RosShared {
   double ph[LPC];       // porosity
   double T[NV][LPC];    // right-hand sides and solutions of the species sweeps
   double inv[NV][LPC];  // reciprocal Thomas pivots
+  double A[NV][LPC];    // sub-diagonal couplings of the forward sweep
+  double E[NV][LPC];    // super-diagonal coupling times reciprocal pivot (back substitution)
   double top[2][NV];    // linearised Dirichlet top-flux terms of the two stages
 };
 
-// Thomas factor of species n = l (lanes l < NV of the group)
+// Thomas factor of species n = l (lanes l < NV of the group): reciprocal
+// pivots, the forward-sweep couplings a_k and the back-substitution
+// couplings c_k / piv_k, so that a sweep is one dependent fma-and-multiply
+// (forward) or multiply-and-add (backward) per layer with operands that do
+// not depend on the chain
 template <int LPC>
 __device__ __forceinline__ void ros_factor(RosShared<LPC> &R, int l, int K, double g, unsigned dirmask)
 {
   if (l < NV) {
     const bool pt = part(l);
-    const double *G = R.G[pt ? 0 : 1];
+    const double *__restrict__ G = R.G[pt ? 0 : 1];
+    const double *__restrict__ W = R.W;
+    const double *__restrict__ ph = R.ph;
+    double *__restrict__ inv = R.inv[l];
+    double *__restrict__ A = R.A[l];
+    double *__restrict__ E = R.E[l];
     const bool dir = (dirmask >> l) & 1u;
     double e = 0.0;
     for (int k = 0; k < K; ++k) {
-      const double qk = pt ? R.ph[k] : 1.0;
+      const double qk = pt ? ph[k] : 1.0;
       const double s = ((k > 0 || dir) ? G[k] : 0.0) + (k < K - 1 ? G[k + 1] : 0.0);
-      const double b = 1.0 + g * (R.W[k] * (s * qk));
-      const double a = k > 0 ? g * ((R.W[k] * G[k]) * (pt ? R.ph[k - 1] : 1.0)) : 0.0;
+      const double b = 1.0 + g * (W[k] * (s * qk));
+      const double a = k > 0 ? g * ((W[k] * G[k]) * (pt ? ph[k - 1] : 1.0)) : 0.0;
       const double piv = k > 0 ? b - a * e : b;
       const double iv = 1.0 / piv;
-      R.inv[l][k] = iv;
-      const double c = k < K - 1 ? g * ((R.W[k] * G[k + 1]) * (pt ? R.ph[k + 1] : 1.0)) : 0.0;
+      const double c = k < K - 1 ? g * ((W[k] * G[k + 1]) * (pt ? ph[k + 1] : 1.0)) : 0.0;
       e = c * iv;
+      inv[k] = iv;
+      A[k] = a;
+      E[k] = e;
     }
   }
 }
 
 // B z = T in place, species n = l
 template <int LPC>
-__device__ __forceinline__ void ros_sweep(RosShared<LPC> &R, int l, int K, double g)
+__device__ __forceinline__ void ros_sweep(RosShared<LPC> &R, int l, int K)
 {
   if (l < NV) {
-    const bool pt = part(l);
-    const double *G = R.G[pt ? 0 : 1];
-    double *T = R.T[l];
-    const double *iv = R.inv[l];
-    double d = 0.0;
-    for (int k = 0; k < K; ++k) {
-      const double a = k > 0 ? g * ((R.W[k] * G[k]) * (pt ? R.ph[k - 1] : 1.0)) : 0.0;
-      d = k > 0 ? (T[k] + a * d) * iv[k] : T[k] * iv[k];
+    double *__restrict__ T = R.T[l];
+    const double *__restrict__ iv = R.inv[l];
+    const double *__restrict__ A = R.A[l];
+    const double *__restrict__ E = R.E[l];
+    double d = T[0] * iv[0];
+    T[0] = d;
+    for (int k = 1; k < K; ++k) {
+      d = (T[k] + A[k] * d) * iv[k];
       T[k] = d;
     }
     for (int k = K - 2; k >= 0; --k) {
-      const double c = g * ((R.W[k] * G[k + 1]) * (pt ? R.ph[k + 1] : 1.0));
-      T[k] = T[k] + (c * iv[k]) * T[k + 1];
+      d = T[k] + E[k] * d;
+      T[k] = d;
     }
   }
 }
@@ -681,7 +694,7 @@ __device__ __forceinline__ void ros_bsolve(RosShared<LPC> &R, int l, int K, bool
   for (int n = 0; n < NV; ++n)
     if (act) R.T[n][l] = r[n];
   __builtin_amdgcn_wave_barrier();
-  ros_sweep<LPC>(R, l, K, g);
+  ros_sweep<LPC>(R, l, K);
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int n = 0; n < NV; ++n) v[n] = act ? R.T[n][l] : 0.0;

@@ -152,11 +152,32 @@ module sedcol_hip
       type(c_ptr), value :: h
       real(c_double), intent(out) :: v
     end function
+    !> per-column spin-up (replaces the presimulation of
+    !> fabm_sediment_component.F90:557-632); cfloor(nvar); flux_ref and
+    !> flux_floor may be c_null_ptr (residual criterion only)
+    integer(c_int) function sedcol_spinup(h, t_start, chunk, t_max, tol, cfloor, flux_ref, flux_rtol, &
+        flux_floor, t_conv, resid, n_unconverged) bind(C, name='sedcol_spinup')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: t_start, chunk, t_max, tol, flux_rtol
+      real(c_double), intent(in) :: cfloor(*)
+      type(c_ptr), value :: flux_ref, flux_floor
+      real(c_double), intent(out) :: t_conv(*), resid(*)
+      integer(c_int32_t), intent(out) :: n_unconverged
+    end function
+    !> per-cell DIC and N2-N release of the resident state (the
+    !> hzg_omexdia_p_denit diagnostic of the standalone output)
+    integer(c_int) function sedcol_rates(h, dic_release, n2_release) bind(C, name='sedcol_rates')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: dic_release(*), n2_release(*)
+    end function
   end interface
 
   public :: sedcol_create, sedcol_destroy, sedcol_set_params, sedcol_set_geometry, sedcol_set_forcing, &
     sedcol_set_forcing_series, sedcol_set_state, sedcol_get_state, sedcol_rhs, sedcol_step, sedcol_run, &
-    sedcol_integrate, sedcol_get_fluxes, sedcol_get_budget, sedcol_get_last_min_dt, sedcol_error_message
+    sedcol_integrate, sedcol_get_fluxes, sedcol_get_budget, sedcol_get_last_min_dt, sedcol_spinup, &
+    sedcol_rates, sedcol_error_message
 
   !> GPU-backed counterpart of type_sed + type_rhs_driver
   type, public :: type_sed_hip
