@@ -237,7 +237,7 @@ def ragged_bench(args):
         "results_bitwise_equal": same,
         "accepted_column_steps": colsteps,
         "autotune": {"tuning": best, "seconds": tune_s, "span_s": args.tune_span,
-                     "ms_table_bin_lanes_workgroup": np.round(table, 3).tolist()},
+                     "ms_table_bin_layers_per_lane_lanes_workgroup": np.round(table, 3).tolist()},
         "rows": allrows,
         "input_generation_s": gen_s,
     }

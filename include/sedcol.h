@@ -165,15 +165,18 @@ typedef struct {
   int32_t concurrent_bins; /* 1 = the layer bins run as concurrent branches */
 } sedcol_schedule;
 
-/* Kernel shape of the adaptive EMP kernel per layer bin (bins hold the wet
- * columns with K <= 16, K <= 32 and K <= 64).  lanes_per_column: 16, 32 or
- * 64 and at least the bin's largest K (one layer per lane); workgroup =
- * lanes_per_column * columns_per_group threads, 64, 128 or 256.  Results are
- * identical for every shape. */
+/* Kernel shape of the adaptive kernel per layer bin (bins hold the wet
+ * columns with K <= 16, K <= 32 and K <= 64).  layers_per_lane: 1, or 2 for
+ * adaptive EMP with product weights in the reference or fast form (other
+ * parameter sets run such a bin with one layer per lane); 0 reads as 1.
+ * lanes_per_column: 16, 32 or 64, at least the bin's largest K divided by
+ * layers_per_lane; workgroup = lanes_per_column * columns_per_group threads,
+ * 64, 128 or 256.  Results are identical for every shape. */
 #define SEDCOL_NBIN 3
 typedef struct {
   int32_t lanes_per_column[SEDCOL_NBIN];
   int32_t columns_per_group[SEDCOL_NBIN];
+  int32_t layers_per_lane[SEDCOL_NBIN];
 } sedcol_tuning;
 
 /* --- lifecycle ---------------------------------------------------------- */
@@ -282,9 +285,9 @@ int sedcol_get_tuning(sedcol_handle *h, sedcol_tuning *t);
  * [t_start, t_start + t_span) for every supported shape of every bin, best
  * of `repeats` runs, from the current state, which is restored after every
  * trial and at the end.  The fastest shape per bin is installed and returned
- * in *best; ms_table (optional, SEDCOL_NBIN x 3 x 3 doubles: bin, lanes
- * 16/32/64, workgroup 64/128/256) receives every time, -1 where a shape does
- * not apply. */
+ * in *best; ms_table (optional, SEDCOL_NBIN x 2 x 3 x 3 doubles: bin,
+ * layers per lane 1/2, lanes 16/32/64, workgroup 64/128/256) receives every
+ * time, -1 where a shape does not apply. */
 int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t repeats,
                     sedcol_tuning *best, double *ms_table);
 

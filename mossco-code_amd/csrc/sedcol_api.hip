@@ -107,6 +107,7 @@ struct sedcol_handle {
   sedcol_schedule sched{0, 0, 1, 0};
   int lpc[3] = {16, 32, 64};  // lanes per column of each bin in the adaptive kernel
   int blk[3] = {256, 256, 256};  // workgroup size of each bin in the adaptive kernel
+  int lpl[3] = {1, 1, 1};        // layers per lane of each bin in the adaptive kernel
   hipStream_t aux[2] = {nullptr, nullptr};  // for concurrent bins
   hipEvent_t evfork = nullptr, evjoin[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;
@@ -294,25 +295,47 @@ int record_of(const sedcol_handle *h, double t)
   return (int)q;
 }
 
-// smallest lanes per column that holds the largest column of bin b
-int bin_min_lpc(const sedcol_handle *h, int b)
+// smallest lanes per column that holds the largest column of bin b with
+// lpl layers per lane (two layers per lane need at least 16 lanes: lanes
+// 0..NV-1 own the top-flux accumulators)
+int bin_min_lpc(const sedcol_handle *h, int b, int lpl = 1)
 {
   int kmax = 1;
   for (int c : h->bins[b]) kmax = std::max(kmax, h->off[c + 1] - h->off[c]);
-  return kmax <= 16 ? 16 : kmax <= 32 ? 32 : 64;
+  const int lanes = (kmax + lpl - 1) / lpl;
+  return lanes <= 16 ? 16 : lanes <= 32 ? 32 : 64;
 }
 
 int check_tuning(const sedcol_handle *h, const sedcol_tuning *t)
 {
   for (int b = 0; b < SEDCOL_NBIN; ++b) {
     const int l = t->lanes_per_column[b], g = t->columns_per_group[b];
+    const int lpl = t->layers_per_lane[b] == 0 ? 1 : t->layers_per_lane[b];
+    if (lpl != 1 && lpl != 2) return fail(SEDCOL_EINVAL, "layers_per_lane must be 1 or 2");
     if (l != 16 && l != 32 && l != 64) return fail(SEDCOL_EINVAL, "lanes_per_column must be 16, 32 or 64");
-    if (l < bin_min_lpc(h, b)) return fail(SEDCOL_EINVAL, "lanes_per_column below the bin's largest column");
+    if (l < bin_min_lpc(h, b, lpl)) return fail(SEDCOL_EINVAL, "lanes_per_column below the bin's largest column");
     const int w = l * g;
     if (g < 1 || (w != 64 && w != 128 && w != 256))
       return fail(SEDCOL_EINVAL, "lanes_per_column * columns_per_group must be 64, 128 or 256");
   }
   return SEDCOL_OK;
+}
+
+// kernel shape bin b runs with under the current parameters: an installed
+// two-layers-per-lane shape falls back to one layer per lane (with enough
+// lanes for the bin) where that kernel does not cover the parameters
+// (linearly implicit stepper, geometric-mean weights, bioturbation profile
+// 3, BcUp 4)
+void effective_shape(const sedcol_handle *h, int b, int *lpc, int *blk, int *lpl)
+{
+  *lpc = h->lpc[b];
+  *blk = h->blk[b];
+  *lpl = h->lpl[b];
+  if (*lpl == 2 && !sc::emp2_supported(h->p)) {
+    *lpl = 1;
+    *lpc = std::max(*lpc, bin_min_lpc(h, b, 1));
+    *blk = std::max(*blk, *lpc);
+  }
 }
 
 void drop_graph(sedcol_handle *h)
@@ -340,7 +363,9 @@ int launch_bins(sedcol_handle *h, int only, int *launches)
       st = h->aux[i - 1];
       HIPOK(hipStreamWaitEvent(st, h->evfork, 0));
     }
-    if (!sc::launch_emp_adaptive(h->lpc[b], h->blk[b], st, h->d_devview, h->d_list[b], (int)h->bins[b].size(), h->p,
+    int lpc, blk, lpl;
+    effective_shape(h, b, &lpc, &blk, &lpl);
+    if (!sc::launch_emp_adaptive(lpc, blk, lpl, st, h->d_devview, h->d_list[b], (int)h->bins[b].size(), h->p,
                                  h->d_params, h->d_ctl, h->d_flag + 1 + b, h->d_flag + 4, h->d_work, h->d_flag))
       return fail(SEDCOL_EINVAL, "unsupported lanes per column / workgroup size");
     ++*launches;
@@ -368,9 +393,12 @@ int capture_graph(sedcol_handle *h)
 {
   drop_graph(h);
   // device queries of the launcher (resident blocks) outside the capture
-  for (int b = 0; b < 3; ++b)
-    if (!h->bins[b].empty() && sc::emp_resident_blocks(h->lpc[b], h->blk[b], h->p) <= 0)
+  for (int b = 0; b < 3; ++b) {
+    int lpc, blk, lpl;
+    effective_shape(h, b, &lpc, &blk, &lpl);
+    if (!h->bins[b].empty() && sc::emp_resident_blocks(lpc, blk, lpl, h->p) <= 0)
       return fail(SEDCOL_EINVAL, "unsupported lanes per column / workgroup size");
+  }
   int launches = 0, rc = SEDCOL_OK;
   HIPOK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   for (int w = 0; w < h->sched.graph_windows && rc == SEDCOL_OK; ++w) {
@@ -1147,6 +1175,7 @@ int sedcol_set_tuning(sedcol_handle *h, const sedcol_tuning *t)
   for (int b = 0; b < SEDCOL_NBIN; ++b) {
     h->lpc[b] = t->lanes_per_column[b];
     h->blk[b] = t->lanes_per_column[b] * t->columns_per_group[b];
+    h->lpl[b] = t->layers_per_lane[b] == 2 ? 2 : 1;
   }
   drop_graph(h);
   return SEDCOL_OK;
@@ -1159,6 +1188,7 @@ int sedcol_get_tuning(sedcol_handle *h, sedcol_tuning *t)
   for (int b = 0; b < SEDCOL_NBIN; ++b) {
     t->lanes_per_column[b] = h->lpc[b];
     t->columns_per_group[b] = h->blk[b] / h->lpc[b];
+    t->layers_per_lane[b] = h->lpl[b];
   }
   return SEDCOL_OK;
 }
@@ -1175,7 +1205,7 @@ int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t rep
   if (repeats < 1 || repeats > 100) return fail(SEDCOL_EINVAL, "repeats outside 1..100");
   static const int kL[3] = {16, 32, 64}, kW[3] = {64, 128, 256};
   if (ms_table)
-    for (int i = 0; i < SEDCOL_NBIN * 9; ++i) ms_table[i] = -1.0;
+    for (int i = 0; i < SEDCOL_NBIN * 18; ++i) ms_table[i] = -1.0;
   // snapshot of everything an integration changes; restored before every
   // trial and at the end, so tuning leaves the handle's state as it was
   const size_t nc = (size_t)h->ncol, ncell = (size_t)h->ncell;
@@ -1204,45 +1234,53 @@ int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t rep
     return SEDCOL_OK;
   };
   const sedcol_schedule saved_sched = h->sched;
-  int saved_lpc[3], saved_blk[3];
+  int saved_lpc[3], saved_blk[3], saved_lpl[3];
   std::memcpy(saved_lpc, h->lpc, sizeof saved_lpc);
   std::memcpy(saved_blk, h->blk, sizeof saved_blk);
+  std::memcpy(saved_lpl, h->lpl, sizeof saved_lpl);
   h->sched = sedcol_schedule{0, 0, 1, 0};  // one launch per trial
-  int best_l[3], best_w[3];
+  int best_l[3], best_w[3], best_p[3];
   std::memcpy(best_l, h->lpc, sizeof best_l);
   std::memcpy(best_w, h->blk, sizeof best_w);
+  std::memcpy(best_p, h->lpl, sizeof best_p);
   for (int b = 0; b < 3 && rc == SEDCOL_OK; ++b) {
     if (h->bins[b].empty()) continue;
-    const int lmin = bin_min_lpc(h, b);
     double best_ms = INFINITY;
-    for (int il = 0; il < 3 && rc == SEDCOL_OK; ++il) {
-      if (kL[il] < lmin) continue;
-      for (int iw = 0; iw < 3 && rc == SEDCOL_OK; ++iw) {
-        if (kW[iw] < kL[il]) continue;
-        h->lpc[b] = kL[il];
-        h->blk[b] = kW[iw];
-        double tmin = INFINITY;
-        for (int r = 0; r < repeats && rc == SEDCOL_OK; ++r) {
-          int launches = 0, flag = 0;
-          if ((rc = restore())) break;
-          if (hipEventRecord(h->ev0, h->stream) != hipSuccess) { rc = fail(SEDCOL_EHIP, "event record"); break; }
-          if ((rc = run_adaptive(h, t_start, t_start + t_span, b, &launches))) break;
-          if (hipEventRecord(h->ev1, h->stream) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess) {
-            rc = fail(SEDCOL_EHIP, "event synchronize");
-            break;
+    for (int ip = 0; ip < 2 && rc == SEDCOL_OK; ++ip) {
+      const int lpl = ip + 1;
+      if (lpl == 2 && !sc::emp2_supported(h->p)) continue;
+      const int lmin = bin_min_lpc(h, b, lpl);
+      for (int il = 0; il < 3 && rc == SEDCOL_OK; ++il) {
+        if (kL[il] < lmin) continue;
+        for (int iw = 0; iw < 3 && rc == SEDCOL_OK; ++iw) {
+          if (kW[iw] < kL[il]) continue;
+          h->lpc[b] = kL[il];
+          h->blk[b] = kW[iw];
+          h->lpl[b] = lpl;
+          double tmin = INFINITY;
+          for (int r = 0; r < repeats && rc == SEDCOL_OK; ++r) {
+            int launches = 0, flag = 0;
+            if ((rc = restore())) break;
+            if (hipEventRecord(h->ev0, h->stream) != hipSuccess) { rc = fail(SEDCOL_EHIP, "event record"); break; }
+            if ((rc = run_adaptive(h, t_start, t_start + t_span, b, &launches))) break;
+            if (hipEventRecord(h->ev1, h->stream) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess) {
+              rc = fail(SEDCOL_EHIP, "event synchronize");
+              break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+            if ((rc = read_flag(h, &flag))) break;
+            if ((rc = flag_status(flag))) break;
+            tmin = std::min(tmin, (double)ms);
           }
-          float ms = 0.f;
-          (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
-          if ((rc = read_flag(h, &flag))) break;
-          if ((rc = flag_status(flag))) break;
-          tmin = std::min(tmin, (double)ms);
-        }
-        if (rc) break;
-        if (ms_table) ms_table[b * 9 + il * 3 + iw] = tmin;
-        if (tmin < best_ms) {
-          best_ms = tmin;
-          best_l[b] = kL[il];
-          best_w[b] = kW[iw];
+          if (rc) break;
+          if (ms_table) ms_table[b * 18 + ip * 9 + il * 3 + iw] = tmin;
+          if (tmin < best_ms) {
+            best_ms = tmin;
+            best_l[b] = kL[il];
+            best_w[b] = kW[iw];
+            best_p[b] = lpl;
+          }
         }
       }
     }
@@ -1254,10 +1292,12 @@ int sedcol_autotune(sedcol_handle *h, double t_start, double t_span, int32_t rep
   if (rc || rc2) {
     std::memcpy(h->lpc, saved_lpc, sizeof saved_lpc);
     std::memcpy(h->blk, saved_blk, sizeof saved_blk);
+    std::memcpy(h->lpl, saved_lpl, sizeof saved_lpl);
     return rc ? rc : rc2;
   }
   std::memcpy(h->lpc, best_l, sizeof best_l);
   std::memcpy(h->blk, best_w, sizeof best_w);
+  std::memcpy(h->lpl, best_p, sizeof best_p);
   drop_graph(h);
   return sedcol_get_tuning(h, best);
 }

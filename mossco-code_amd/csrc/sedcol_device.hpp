@@ -781,4 +781,137 @@ __device__ __forceinline__ double pfactor(const double (&c)[NV], const double (&
   else return psolve<LPC, STAGE2>(c, g, den, h, active, zero_sink, newton);
 }
 
+// ---------------------------------------------------------------------------
+// two layers per lane
+// ---------------------------------------------------------------------------
+//
+// Sub-layer s of lane l holds layer 2l + s.  The right-hand side takes the
+// neighbour above sub-layer 0 from the previous lane's sub-layer 1 and the
+// flux below sub-layer 1 from the next lane's sub-layer 0; the other two
+// neighbours are the lane's own.  Layer reductions combine the two
+// sub-layers first, which is the first level of the pairwise tree over the
+// layers (lower layer first), and then run the lane butterfly: every
+// reduction is the tree of the one-layer-per-lane kernels, bit for bit.
+
+// get_rhs for the lane's two layers, reference and fast forms (bioturbation
+// profile 3 and BcUp 4 need the one-layer kernels); the callback receives
+// (s, n, rhs, F) with F the flux through sub-layer s's upper interface
+template <int ARITH, class CB>
+__device__ __forceinline__ void column_rhs2(const sedcol_params &p, const Lane (&L)[2], const double *sbc,
+                                            const double (&c)[2][NV], double (&sC)[2], double (&sN)[2], CB &&cb)
+{
+  static_assert(ARITH == ARITH_REF || ARITH == ARITH_FAST, "two layers per lane: reference and fast forms");
+  __builtin_amdgcn_wave_barrier();
+  double R0[NV], R1[NV];
+  kinetics<ARITH>(p, L[0].fT, c[0], R0, sC[0], sN[0]);
+  kinetics<ARITH>(p, L[1].fT, c[1], R1, sC[1], sN[1]);
+  const bool top = L[0].l == 0;
+  const bool bottom0 = L[0].l == L[0].K - 1, bottom1 = L[1].l == L[1].K - 1;
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    const bool pt = part(n);
+    const double x0 = pt ? c[0][n] * L[0].phi : c[0][n];
+    const double x1 = pt ? c[1][n] * L[1].phi : c[1][n];
+    const double xup = lg_up(x1);
+    const int bc = pt ? p.bcup_part : p.bcup_dis;
+    if (ARITH == ARITH_REF) {
+      const double D0 = pt ? L[0].Dbio : L[0].Ddis;
+      const double D1 = pt ? L[1].Dbio : L[1].Ddis;
+      double F0;
+      if (top) F0 = (bc == 1 || bc == 4) ? sbc[NV + n] : bc == 2 ? -((D0 * (x0 - sbc[n])) / L[0].dz) : 0.0;
+      else F0 = -((D0 * (x0 - xup)) / L[0].dzc);
+      const double F1 = -((D1 * (x1 - x0)) / L[1].dzc);
+      double Fdn0 = F1;
+      if (bottom0) Fdn0 = p.bcdown == 2 ? -((D0 * (0.0 - x0)) / L[0].dz) : 0.0;
+      double Fdn1 = lg_dn(F0);
+      if (bottom1) Fdn1 = p.bcdown == 2 ? -((D1 * (0.0 - x1)) / L[1].dz) : 0.0;
+      const double VF0 = pt ? (1.0 - L[0].phi) : L[0].phi;
+      const double VF1 = pt ? (1.0 - L[1].phi) : L[1].phi;
+      const double dC0 = (F0 - Fdn0) / (VF0 * L[0].dz);
+      const double dC1 = (F1 - Fdn1) / (VF1 * L[1].dz);
+      const double Q0 = pt ? (dC0 * (1.0 - L[0].phi)) / L[0].phi : dC0;
+      const double Q1 = pt ? (dC1 * (1.0 - L[1].phi)) / L[1].phi : dC1;
+      cb(0, n, R0[n] + Q0, F0);
+      cb(1, n, R1[n] + Q1, F1);
+    } else {
+      const double G0 = pt ? L[0].Gb : L[0].Gd;
+      const double G1 = pt ? L[1].Gb : L[1].Gd;
+      double F0;
+      if (top) F0 = bc == 1 ? sbc[NV + n] : bc == 2 ? -(G0 * (x0 - sbc[n])) : 0.0;
+      else F0 = -(G0 * (x0 - xup));
+      const double F1 = -(G1 * (x1 - x0));
+      double Fdn0 = F1;
+      if (bottom0) Fdn0 = 0.0;
+      double Fdn1 = lg_dn(F0);
+      if (bottom1) Fdn1 = 0.0;
+      cb(0, n, R0[n] + (F0 - Fdn0) * L[0].W, F0);
+      cb(1, n, R1[n] + (F1 - Fdn1) * L[1].W, F1);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// psolve for two layers per lane (product weights), the same operation
+// sequence per layer and the same reduction tree
+template <int LPC, bool STAGE2>
+__device__ __forceinline__ double psolve2(const double (&c)[2][NV], const double (&g)[2][NV],
+                                          const double (&den)[2][NV], double h, const bool (&active)[2],
+                                          int64_t &zero_sink, int64_t &newton)
+{
+  double a[2][NV];
+  double Sl[2] = {1.0, 1.0};
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      double av = 0.0;
+      if (active[s] && g[s][n] < 0.0) {
+        if (c[s][n] > 0.0) {
+          av = (h * g[s][n]) / c[s][n];
+          if (STAGE2 && den[s][n] > 0.0) Sl[s] = Sl[s] * (c[s][n] / den[s][n]);
+        } else {
+          zero_sink += 1;
+        }
+      }
+      a[s][n] = av;
+    }
+  const double Sc = STAGE2 ? lg_prod<LPC>(Sl[0] * Sl[1]) : 1.0;
+  double Ds[2] = {0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int n = 0; n < NV; ++n) Ds[s] = Ds[s] + a[s][n];
+  const double D0 = lg_sum<LPC>(Ds[0] + Ds[1]);
+  newton += 1;
+  double pp = 0.0 - Sc / (Sc * D0 - 1.0);
+  if (!(pp > 0.0)) return 0.0;
+  if (pp <= 0x1p-20 * pp) return (!STAGE2 && pp > 1.0) ? 1.0 : pp;
+  double dprev = pp;
+  for (int it = 1; it < 60; ++it) {
+    newton += 1;
+    double Ps[2] = {1.0, 1.0}, Dq[2] = {0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        const double f = fma(a[s][n], pp, 1.0);
+        Dq[s] = fma(Dq[s], f, Ps[s] * a[s][n]);
+        Ps[s] = Ps[s] * f;
+      }
+    double D = Dq[0] * Ps[1] + Ps[0] * Dq[1];
+    double P = Ps[0] * Ps[1];
+    lg_pd<LPC>(P, D);
+    const double hv = Sc * P - pp;
+    const double hp = Sc * D - 1.0;
+    const double pn = pp - hv / hp;
+    if (!(pn > pp)) break;
+    const double dl = pn - pp;
+    const bool conv = dl <= 0x1p-20 * pn || (dl <= 0x1p-10 * pn && (dl * dl) * dl <= (0x1p-30 * pn) * (dprev * dprev));
+    dprev = dl;
+    pp = pn;
+    if (conv) break;
+  }
+  return (!STAGE2 && pp > 1.0) ? 1.0 : pp;
+}
+
 }  // namespace sc

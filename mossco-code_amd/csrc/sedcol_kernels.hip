@@ -591,6 +591,280 @@ __global__ __launch_bounds__(BLK, SEDCOL_EMP_MIN_WAVES) void k_emp_adaptive(
 }
 
 // ---------------------------------------------------------------------------
+// adaptive EMP with two layers per lane
+// ---------------------------------------------------------------------------
+//
+// k_emp_adaptive with lane l of a group holding layers 2l and 2l+1
+// (column_rhs2, psolve2): a group of LPC lanes takes columns of up to 2 LPC
+// layers, so a wave holds twice the columns at the same lane count per
+// layer.  Product weights, reference and fast forms.  Every operation per
+// layer and every reduction tree are those of k_emp_adaptive, so the
+// results are bitwise the same.  LPC >= 16: lanes 0..NV-1 own the top-flux
+// accumulators.
+
+// one wave per SIMD: two layers of state and stage values per lane exceed
+// the 256 registers of two waves
+#ifndef SEDCOL_EMP2_MIN_WAVES
+#define SEDCOL_EMP2_MIN_WAVES 1
+#endif
+
+__device__ __forceinline__ void setup_lane2(const Dev &d, int c, int l, Lane (&L)[2], Geo (&G)[2],
+                                            long long (&cell)[2])
+{
+  setup_lane(d, c, 2 * l, L[0], G[0], cell[0]);
+  setup_lane(d, c, 2 * l + 1, L[1], G[1], cell[1]);
+}
+
+template <int ARITH>
+__device__ __forceinline__ void load_forcing2(const Dev &d, const sedcol_params &p, int c, int r, int l,
+                                              const Geo (&G)[2], Lane (&L)[2], double *sbc)
+{
+  if (l < NV) {
+    sbc[l] = d.cbw[((long long)r * NV + l) * d.ncol + c];
+    sbc[NV + l] = d.fup[(long long)r * d.fup_rec_stride + (long long)l * d.ncol + c];
+  }
+  const double T = d.temp[(long long)r * d.ncol + c];
+  diffusivities<ARITH>(p, G[0].B, T, G[0].phiI, G[0].bfac, L[0]);
+  diffusivities<ARITH>(p, G[1].B, T, G[1].phiI, G[1].bfac, L[1]);
+}
+
+template <int LPC, int ARITH, int BLK>
+__global__ __launch_bounds__(BLK, SEDCOL_EMP2_MIN_WAVES) void k_emp_adaptive2(
+    const Dev *dptr, const int *list, int nlist, const sedcol_params *__restrict__ pptr,
+    const EmpCtl *__restrict__ cptr, int *next, int *pending, int *work, int *errflag)
+{
+  static_assert(LPC >= 16, "lanes 0..NV-1 own the top-flux accumulators");
+  const Dev &d = *dptr;
+  const sedcol_params &p = *pptr;
+  const EmpCtl &ctl = *cptr;
+  constexpr int G = BLK / LPC;
+  __shared__ double sbc[G][2 * NV];
+  __shared__ double sF[G][2][NV];
+  __shared__ int sslot[G];
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const bool own = l < NV;
+  double(*F)[NV] = sF[g];
+  const double *bc = sbc[g];
+  const int r1 = ctl.r1;
+
+  int c = 0, r = 0;
+  Lane L[2];
+  Geo Gm[2];
+  long long cell[2] = {0, 0};
+  bool act[2] = {false, false};
+  double x[2][NV];
+  // per-cell budget sums and compensations of both sub-layers, and the
+  // lane's top-flux accumulators, in LDS as in k_emp_adaptive
+  __shared__ double scell[4][2][BLK];
+  __shared__ double scol[3][BLK];
+  double &fint = scol[0][threadIdx.x], &fintc = scol[1][threadIdx.x], &flast = scol[2][threadIdx.x];
+  sedcol_coldiag dg;
+  int64_t zs = 0;
+  long long attempts = 0;
+  int wn = 0;
+  double dt = 0.0, rem = 0.0;
+  bool have = false;
+
+  for (;;) {
+    if (!have) {
+      if (l == 0) sslot[g] = atomicAdd(next, 1);
+      __builtin_amdgcn_wave_barrier();
+      const int sl = sslot[g];
+      __builtin_amdgcn_wave_barrier();
+      if (sl >= nlist) break;
+      c = list[sl];
+      if (ctl.resume) {
+        r = d.wrec[c];
+        if (r > r1) continue;
+        rem = d.wrem[c];
+        attempts = d.watt[c];
+      } else {
+        r = ctl.r0 - 1;
+        rem = 0.0;
+        attempts = 0;
+      }
+      setup_lane2(d, c, l, L, Gm, cell);
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        act[s] = L[s].active;
+#pragma unroll
+        for (int n = 0; n < NV; ++n) x[s][n] = act[s] ? d.conc[n * d.ncell + cell[s]] : 0.0;
+        scell[0][s][threadIdx.x] = act[s] ? d.cellacc[cell[s]] : 0.0;
+        scell[1][s][threadIdx.x] = act[s] ? d.cellacc[d.ncell + cell[s]] : 0.0;
+        scell[2][s][threadIdx.x] = act[s] ? d.cellacc[5 * d.ncell + cell[s]] : 0.0;
+        scell[3][s][threadIdx.x] = act[s] ? d.cellacc[6 * d.ncell + cell[s]] : 0.0;
+      }
+      fint = own ? d.colacc[(long long)l * d.ncol + c] : 0.0;
+      fintc = own ? d.colacc[(long long)(NV + l) * d.ncol + c] : 0.0;
+      flast = own ? d.flux[(long long)l * d.ncol + c] : 0.0;
+      dg = d.diag[c];
+      dt = dg.dt_next;
+      if (!(dt > 0.0)) dt = p.emp_dt_max;
+      zs = 0;
+      wn = 0;
+      have = true;
+      if (rem > 0.0) {
+        load_forcing2<ARITH>(d, p, c, r, l, Gm, L, sbc[g]);
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    bool finish = false;
+    if (!(rem > 0.0)) {
+      ++r;
+      if (r > r1) {
+        finish = true;
+      } else {
+        const double lo = (r == 0) ? -INFINITY : d.t0 + (double)r * d.dt_rec;
+        const double hi = (r == d.nrec - 1) ? INFINITY : d.t0 + (double)(r + 1) * d.dt_rec;
+        const double a = ctl.t_start > lo ? ctl.t_start : lo;
+        const double b = ctl.t_end < hi ? ctl.t_end : hi;
+        rem = b - a;
+        load_forcing2<ARITH>(d, p, c, r, l, Gm, L, sbc[g]);
+        __builtin_amdgcn_wave_barrier();
+        continue;
+      }
+    }
+    bool pause = false;
+    if (!finish) {
+      if (attempts >= ctl.max_attempts) {
+        dg.status = 2;
+        if (l == 0) atomicOr(errflag, 2);
+        finish = true;
+      } else if (ctl.window > 0 && wn >= ctl.window) {
+        pause = true;
+      }
+    }
+    if (!finish && !pause) {
+      ++attempts;
+      ++wn;
+      const bool clipped = !(dt < rem);
+      const double h = clipped ? rem : dt;
+      double f0[2][NV], c1[2][NV];
+      double s0C[2], s0N[2], s1C[2], s1N[2];
+      column_rhs2<ARITH>(p, L, bc, x, s0C, s0N, [&](int s, int n, double rr, double Fv) {
+        f0[s][n] = rr;
+        if (l == 0 && s == 0) F[0][n] = Fv;
+      });
+      const double p1 = psolve2<LPC, false>(x, f0, x, h, act, zs, dg.newton);
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+          const double y = x[s][n] + (h * p1) * f0[s][n];
+          c1[s][n] = y < 0.0 ? 0.0 : y;
+        }
+      column_rhs2<ARITH>(p, L, bc, c1, s1C, s1N, [&](int s, int n, double rr, double Fv) {
+        f0[s][n] = (f0[s][n] + rr) * 0.5;
+        if (l == 0 && s == 0) F[1][n] = Fv;
+      });
+      const double p2 = psolve2<LPC, true>(x, f0, c1, h, act, zs, dg.newton);
+      double el[2];
+      bool bad = false;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        double bn = 0.0, bd = 1.0;
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+          const double y = x[s][n] + (h * p2) * f0[s][n];
+          const double yp = y < 0.0 ? 0.0 : y;
+          const double sc = p.emp_atol + p.emp_rtol * (yp > x[s][n] ? yp : x[s][n]);
+          const double dv = fabs(yp - c1[s][n]);
+          bad |= act[s] && ((dv != dv) || (sc != sc));
+          if (dv * bd > bn * sc) {
+            bn = dv;
+            bd = sc;
+          }
+        }
+        el[s] = bn / bd;
+        bad |= act[s] && (el[s] != el[s]);
+        if (!act[s]) el[s] = 0.0;
+      }
+      // the first tree level (lower layer first), then the lane butterfly
+      const double elp = (el[1] > el[0] || el[1] != el[1]) ? el[1] : el[0];
+      const double err = lg_max<LPC>(elp);
+      if (group_any<LPC>(bad)) {
+        dg.status = 1;
+        if (l == 0) atomicOr(errflag, 1);
+        finish = true;
+      } else {
+        const bool accept = (err <= 1.0) || !(h > p.emp_dt_min);
+        double fac = err > 0.0 ? 0.9 * drsqrt(err) : 5.0;
+        if (fac < 0.2) fac = 0.2;
+        if (fac > 5.0) fac = 5.0;
+        if (p1 < dg.p_min) dg.p_min = p1;
+        if (p2 < dg.p_min) dg.p_min = p2;
+        if (accept) {
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+              double y = x[s][n] + (h * p2) * f0[s][n];
+              if (y < 0.0) {
+                if (elem(n) >= 0 && act[s])
+                  kadd(d.cellacc[(2 + elem(n)) * d.ncell + cell[s]], d.cellacc[(7 + elem(n)) * d.ncell + cell[s]],
+                       0.0 - y);
+                y = 0.0;
+              }
+              x[s][n] = y;
+            }
+          if (own) {
+            kadd(fint, fintc, (h * p2) * ((F[0][l] + F[1][l]) * 0.5));
+            flast = F[1][l];
+          }
+#pragma unroll
+          for (int s = 0; s < 2; ++s) {
+            kadd(scell[0][s][threadIdx.x], scell[2][s][threadIdx.x], (h * p2) * ((s0C[s] + s1C[s]) * 0.5));
+            kadd(scell[1][s][threadIdx.x], scell[3][s][threadIdx.x], (h * p2) * ((s0N[s] + s1N[s]) * 0.5));
+          }
+          rem = rem - h;
+          dg.accepted += 1;
+          if (err > 1.0) dg.forced += 1;
+          dg.dt_last = h;
+          if (h < dg.dt_minacc) dg.dt_minacc = h;
+          const double prop = h * fac;
+          dt = (clipped && dt > prop) ? dt : prop;
+        } else {
+          dg.rejected += 1;
+          dt = h * fac;
+        }
+        if (dt < p.emp_dt_min) dt = p.emp_dt_min;
+        if (dt > p.emp_dt_max) dt = p.emp_dt_max;
+      }
+    }
+    if (finish || pause) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        if (act[s]) {
+#pragma unroll
+          for (int n = 0; n < NV; ++n) d.conc[n * d.ncell + cell[s]] = x[s][n];
+          d.cellacc[cell[s]] = scell[0][s][threadIdx.x];
+          d.cellacc[d.ncell + cell[s]] = scell[1][s][threadIdx.x];
+          d.cellacc[5 * d.ncell + cell[s]] = scell[2][s][threadIdx.x];
+          d.cellacc[6 * d.ncell + cell[s]] = scell[3][s][threadIdx.x];
+        }
+      }
+      if (own) {
+        d.colacc[(long long)l * d.ncol + c] = fint;
+        d.colacc[(long long)(NV + l) * d.ncol + c] = fintc;
+        d.flux[(long long)l * d.ncol + c] = flast;
+      }
+      const double zsum = lg_sum<LPC>((double)zs);
+      if (l == 0) {
+        dg.zero_sink += (int64_t)zsum;
+        dg.dt_next = dt;
+        d.diag[c] = dg;
+        work[c] = attempts < 0x7fffffffll ? (int)attempts : 0x7fffffff;
+        d.wrec[c] = pause ? r : r1 + 1;
+        d.wrem[c] = rem;
+        d.watt[c] = attempts;
+        if (pause) atomicAdd(pending, 1);
+      }
+      have = false;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // linearly implicit adaptive stepper (method 7): ROS2 as a W-method
 // ---------------------------------------------------------------------------
 //
@@ -1345,7 +1619,8 @@ void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *
 // blocks of a work-queue kernel that are resident at once on the current
 // device, cached per device and kernel instance (threads that race on a
 // first query store the same value).  PW = 2 selects the linearly implicit
-// kernel k_ros_adaptive, otherwise k_emp_adaptive with Patankar weights PW.
+// kernel k_ros_adaptive, PW = 3 k_emp_adaptive2 (two layers per lane),
+// otherwise k_emp_adaptive with Patankar weights PW.
 template <int LPC, int AR, int BLK, int PW>
 static int emp_resident()
 {
@@ -1362,6 +1637,8 @@ static int emp_resident()
   hipError_t e;
   if constexpr (PW == 2)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ros_adaptive<LPC, AR, BLK>, BLK, 0);
+  else if constexpr (PW == 3)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive2<LPC, AR, BLK>, BLK, 0);
   else
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_emp_adaptive<LPC, AR, BLK, PW>, BLK, 0);
   if (e != hipSuccess) per_cu = 1;
@@ -1398,12 +1675,20 @@ static bool emp_shape_ok(int lpc, int blk)
 
 static bool ros_selected(const sedcol_params &p) { return p.adaptive_method == SEDCOL_ROS2_ADAPTIVE; }
 
-int emp_resident_blocks(int lpc, int blk, const sedcol_params &p)
+bool emp2_supported(const sedcol_params &p)
+{
+  return !ros_selected(p) && p.emp_weights == 0 && arith_form(p) != ARITH_EXT;
+}
+
+int emp_resident_blocks(int lpc, int blk, int lpl, const sedcol_params &p)
 {
   if (!emp_shape_ok(lpc, blk)) return 0;
+  if (lpl == 2 && (lpc < 16 || !emp2_supported(p))) return 0;
   int out = 0;
   auto body = [&]() -> bool {
-    if (ros_selected(p)) {
+    if (lpl == 2) {
+      SC_DISPATCH_ROS_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, out = emp_resident<LPC, AR, BLK, 3>())));
+    } else if (ros_selected(p)) {
       SC_DISPATCH_ROS_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, out = emp_resident<LPC, AR, BLK, 2>())));
     } else {
       SC_DISPATCH_PW(p, SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, out = emp_resident<LPC, AR, BLK, PW>()))));
@@ -1414,11 +1699,23 @@ int emp_resident_blocks(int lpc, int blk, const sedcol_params &p)
   return out;
 }
 
-bool launch_emp_adaptive(int lpc, int blk, hipStream_t st, const Dev *d_dev, const int *list, int n,
+bool launch_emp_adaptive(int lpc, int blk, int lpl, hipStream_t st, const Dev *d_dev, const int *list, int n,
                          const sedcol_params &p, const sedcol_params *d_params, const EmpCtl *d_ctl, int *next,
                          int *pending, int *work, int *errflag)
 {
   if (!emp_shape_ok(lpc, blk)) return false;
+  if (lpl == 2) {
+    if (lpc < 16 || !emp2_supported(p)) return false;
+    SC_DISPATCH_ROS_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, {
+      const int G = BLK / LPC;
+      const unsigned need = (unsigned)((n + G - 1) / G);
+      const unsigned cap = (unsigned)emp_resident<LPC, AR, BLK, 3>();
+      hipLaunchKernelGGL((k_emp_adaptive2<LPC, AR, BLK>), dim3(need < cap ? need : cap), dim3(BLK), 0, st, d_dev,
+                         list, n, d_params, d_ctl, next, pending, work, errflag);
+    })));
+    return true;
+  }
+  if (lpl != 1) return false;
   if (ros_selected(p)) {
     SC_DISPATCH_ROS_ARITH(p, SC_DISPATCH_LPC(lpc, SC_DISPATCH_BLK(blk, {
       const int G = BLK / LPC;

@@ -56,12 +56,18 @@ void launch_fixed(int lpc, int method, hipStream_t st, const Dev &d, const int *
 // column stopped by the window limit; work[c] receives the attempts column c
 // has taken since the start of the span.  d_dev / d_params: device copies of
 // the domain view and the parameters (p is the host copy, used to pick the
-// kernel instance).  Returns false for an unsupported (lpc, blk).
-bool launch_emp_adaptive(int lpc, int blk, hipStream_t st, const Dev *d_dev, const int *list, int n,
+// kernel instance).  lpl: layers per lane, 1 (k_emp_adaptive / the
+// linearly implicit kernel) or 2 (k_emp_adaptive2: adaptive EMP with product
+// weights in the reference or fast form, lpc >= 16).  Returns false for an
+// unsupported (lpc, blk, lpl).
+bool launch_emp_adaptive(int lpc, int blk, int lpl, hipStream_t st, const Dev *d_dev, const int *list, int n,
                          const sedcol_params &p, const sedcol_params *d_params, const EmpCtl *d_ctl, int *next,
                          int *pending, int *work, int *errflag);
-// blocks of that kernel resident at once on the current device
-int emp_resident_blocks(int lpc, int blk, const sedcol_params &p);
+// blocks of that kernel resident at once on the current device (0: shape
+// not supported)
+int emp_resident_blocks(int lpc, int blk, int lpl, const sedcol_params &p);
+// whether parameters p can run with two layers per lane
+bool emp2_supported(const sedcol_params &p);
 // start position of every listed column: record r0 - 1, nothing left in it,
 // no attempts
 void launch_emp_begin(hipStream_t st, const int *list, int n, int r0, int *wrec, double *wrem, long long *watt);

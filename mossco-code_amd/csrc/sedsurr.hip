@@ -56,7 +56,7 @@ __device__ inline float gelu_grad(float z)
   return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGc * (1.0f + 3.0f * kGa * z * z);
 }
 
-constexpr int BM = 64, BN = 64, BK = 32;
+constexpr int BM = 64, BN = 64, BK = 64;
 enum { EPI_F32 = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_GELU_GRAD = 3 };
 
 struct Epi {
@@ -70,21 +70,26 @@ struct Epi {
   int ldz;
 };
 
-// four consecutive bf16 (8 bytes) of a row-major array, zero outside
-// [0, lim) of the fastest index; one 8-byte load when all four are inside
+// eight consecutive bf16 (16 bytes) of a row-major array, zero outside
+// [0, lim) of the fastest index; one 16-byte load when all eight are inside
 // and the address is aligned
-__device__ inline void load4(const bf16 *__restrict__ p, int i0, int lim, bool row_ok, bf16 (&v)[4])
+__device__ inline uint4 load8(const bf16 *__restrict__ p, int i0, int lim, bool row_ok)
 {
-  if (row_ok && i0 + 3 < lim && ((reinterpret_cast<uintptr_t>(p + i0) & 7u) == 0)) {
-    const uint2 w = *reinterpret_cast<const uint2 *>(p + i0);
-    v[0] = (bf16)(w.x & 0xffffu);
-    v[1] = (bf16)(w.x >> 16);
-    v[2] = (bf16)(w.y & 0xffffu);
-    v[3] = (bf16)(w.y >> 16);
-  } else {
+  if (row_ok && i0 + 7 < lim && ((reinterpret_cast<uintptr_t>(p + i0) & 15u) == 0))
+    return *reinterpret_cast<const uint4 *>(p + i0);
+  unsigned w[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = (row_ok && i0 + j < lim) ? p[i0 + j] : (bf16)0;
+  for (int j = 0; j < 4; ++j) {
+    const unsigned lo = (row_ok && i0 + 2 * j < lim) ? p[i0 + 2 * j] : 0u;
+    const unsigned hi = (row_ok && i0 + 2 * j + 1 < lim) ? p[i0 + 2 * j + 1] : 0u;
+    w[j] = lo | (hi << 16);
   }
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+__device__ inline bf16 elem8(const uint4 &w, int j)
+{
+  const unsigned u = j < 2 ? w.x : j < 4 ? w.y : j < 6 ? w.z : w.w;
+  return (bf16)((j & 1) ? (u >> 16) : (u & 0xffffu));
 }
 
 // C (M x N) = op(A) (M x K) op(B) (K x N); op(A)(m,k) = TA ? A[k lda + m] :
@@ -96,14 +101,18 @@ __device__ inline void load4(const bf16 *__restrict__ p, int i0, int lim, bool r
 // Both tiles sit in LDS with k contiguous (A as [m][k], B as [n][k]), so a
 // lane's MFMA operand (8 consecutive k of one row or column) is one 16-byte
 // LDS read.  v_mfma_f32_32x32x16_bf16: lane l supplies row (A) / column (B)
-// l % 32 at k = 8 (l / 32) + 0..7.
+// l % 32 at k = 8 (l / 32) + 0..7.  The k loop is software pipelined: the
+// next 64-wide k tile is loaded from global memory into registers while the
+// MFMAs run on the current one, and the LDS tiles are double buffered, so
+// one barrier per k tile suffices.  The MFMAs accumulate k in increasing
+// order of 16-wide steps, as for any tile width.
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, const bf16 *__restrict__ A, int lda,
                                               const bf16 *__restrict__ B, int ldb, Epi e)
 {
-  constexpr int LDK = BK + 8;  // row pitch of the tiles: 80 bytes, 16-byte aligned rows
-  __shared__ __attribute__((aligned(16))) bf16 sA[BM][LDK];
-  __shared__ __attribute__((aligned(16))) bf16 sB[BN][LDK];
+  constexpr int LDK = BK + 8;  // row pitch of the tiles: 144 bytes, 16-byte aligned rows
+  __shared__ __attribute__((aligned(16))) bf16 sA[2][BM][LDK];
+  __shared__ __attribute__((aligned(16))) bf16 sB[2][BN][LDK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -113,42 +122,66 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, c
 #pragma unroll
   for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
   const int r = lane & 31, q8 = (lane >> 5) * 8;
-  for (int k0 = kb0; k0 < kend; k0 += BK) {
+
+  // this thread's share of a 64 x 64 tile of each operand: eight
+  // consecutive elements at idx = 8 tid + 2048 h, h = 0, 1
+  uint4 ra[2], rb[2];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int idx = tid * 4 + 1024 * h;  // four consecutive elements of the 64 x 32 tile
-      bf16 v[4];
+      const int idx = tid * 8 + 2048 * h;
       if (TA) {  // A stored k-major: m fastest
         const int m = idx % BM, k = idx / BM;
-        load4(A + (size_t)(k0 + k) * lda, bm + m, M, k0 + k < kend, v);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sA[m + j][k] = v[j];
+        ra[h] = load8(A + (size_t)(k0 + k) * lda, bm + m, M, k0 + k < kend);
       } else {
         const int m = idx / BK, k = idx % BK;
-        load4(A + (size_t)(bm + m) * lda, k0 + k, kend, bm + m < M, v);
-        *reinterpret_cast<uint2 *>(&sA[m][k]) =
-            make_uint2((unsigned)v[0] | ((unsigned)v[1] << 16), (unsigned)v[2] | ((unsigned)v[3] << 16));
+        ra[h] = load8(A + (size_t)(bm + m) * lda, k0 + k, kend, bm + m < M);
       }
       if (TB) {  // B stored n-major: k fastest
-        const int kb = idx % BK, n = idx / BK;
-        load4(B + (size_t)(bn + n) * ldb, k0 + kb, kend, bn + n < N, v);
-        *reinterpret_cast<uint2 *>(&sB[n][kb]) =
-            make_uint2((unsigned)v[0] | ((unsigned)v[1] << 16), (unsigned)v[2] | ((unsigned)v[3] << 16));
+        const int n = idx / BK, kb = idx % BK;
+        rb[h] = load8(B + (size_t)(bn + n) * ldb, k0 + kb, kend, bn + n < N);
       } else {
         const int kb = idx / BN, n = idx % BN;
-        load4(B + (size_t)(k0 + kb) * ldb, bn + n, N, k0 + kb < kend, v);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sB[n + j][kb] = v[j];
+        rb[h] = load8(B + (size_t)(k0 + kb) * ldb, bn + n, N, k0 + kb < kend);
       }
     }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int idx = tid * 8 + 2048 * h;
+      if (TA) {
+        const int m = idx % BM, k = idx / BM;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sA[buf][m + j][k] = elem8(ra[h], j);
+      } else {
+        const int m = idx / BK, k = idx % BK;
+        *reinterpret_cast<uint4 *>(&sA[buf][m][k]) = ra[h];
+      }
+      if (TB) {
+        const int n = idx / BK, kb = idx % BK;
+        *reinterpret_cast<uint4 *>(&sB[buf][n][kb]) = rb[h];
+      } else {
+        const int kb = idx / BN, n = idx % BN;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sB[buf][n + j][kb] = elem8(rb[h], j);
+      }
+    }
+  };
+
+  int buf = 0;
+  if (kb0 < kend) fetch(kb0);
+  for (int k0 = kb0; k0 < kend; k0 += BK) {
+    stage(buf);
     __syncthreads();
+    if (k0 + BK < kend) fetch(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 16) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sA[wm + r][kk + q8]);
-      const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sB[wn + r][kk + q8]);
+      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sA[buf][wm + r][kk + q8]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sB[buf][wn + r][kk + q8]);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
-    __syncthreads();
+    buf ^= 1;
   }
   // accumulator layout of the 32 x 32 result: lane holds column (lane & 31),
   // register v holds row 8 (v / 4) + 4 (lane / 32) + v % 4

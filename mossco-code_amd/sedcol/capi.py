@@ -84,7 +84,12 @@ ZW_DEFAULTS = (0.11, 0.2, 0.6, 0.22, 1.334, 1000.0)
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("lanes_per_column", ctypes.c_int32 * 3), ("columns_per_group", ctypes.c_int32 * 3)]
+    _fields_ = [("lanes_per_column", ctypes.c_int32 * 3), ("columns_per_group", ctypes.c_int32 * 3),
+                ("layers_per_lane", ctypes.c_int32 * 3)]
+
+    def as_dict(self) -> dict:
+        return {"lanes_per_column": list(self.lanes_per_column), "columns_per_group": list(self.columns_per_group),
+                "layers_per_lane": list(self.layers_per_lane)}
 
 
 DIAG_DTYPE = np.dtype([("accepted", np.int64), ("rejected", np.int64), ("forced", np.int64),
@@ -330,26 +335,27 @@ class SedimentColumns:
         check(lib().sedcol_get_schedule(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in Schedule._fields_}
 
-    def set_tuning(self, lanes_per_column, columns_per_group):
-        t = Tuning((ctypes.c_int32 * 3)(*lanes_per_column), (ctypes.c_int32 * 3)(*columns_per_group))
+    def set_tuning(self, lanes_per_column, columns_per_group, layers_per_lane=(1, 1, 1)):
+        t = Tuning((ctypes.c_int32 * 3)(*lanes_per_column), (ctypes.c_int32 * 3)(*columns_per_group),
+                   (ctypes.c_int32 * 3)(*layers_per_lane))
         check(lib().sedcol_set_tuning(self._h, ctypes.byref(t)))
 
     def get_tuning(self) -> dict:
         t = Tuning()
         check(lib().sedcol_get_tuning(self._h, ctypes.byref(t)))
-        return {"lanes_per_column": list(t.lanes_per_column), "columns_per_group": list(t.columns_per_group)}
+        return t.as_dict()
 
     def autotune(self, t_start, t_span, repeats=2):
         """Time every kernel shape per layer bin, install the fastest.
 
-        Returns (tuning dict, ms table of shape (3 bins, 3 lanes 16/32/64,
-        3 workgroups 64/128/256), -1 where a shape does not apply)."""
+        Returns (tuning dict, ms table of shape (3 bins, 2 layers per lane
+        1/2, 3 lanes 16/32/64, 3 workgroups 64/128/256), -1 where a shape
+        does not apply)."""
         t = Tuning()
-        table = np.empty((NBIN, 3, 3))
+        table = np.empty((NBIN, 2, 3, 3))
         check(lib().sedcol_autotune(self._h, float(t_start), float(t_span), int(repeats), ctypes.byref(t),
                                     _pd(table)))
-        return ({"lanes_per_column": list(t.lanes_per_column), "columns_per_group": list(t.columns_per_group)},
-                table)
+        return t.as_dict(), table
 
     def spinup(self, t_start, chunk, t_max, tol, cfloor, flux_ref=None, flux_rtol=0.01, flux_floor=None):
         """Per-column adaptive-EMP spin-up; returns (t_conv, resid,

@@ -31,7 +31,7 @@ def test_struct_sizes():
     assert ctypes.sizeof(capi.ColDiag) == 9 * 8 + 2 * 4
     assert ctypes.sizeof(capi.Desc) == 4 * 4 + 2 * 8
     assert ctypes.sizeof(capi.Schedule) == 4 * 4
-    assert ctypes.sizeof(capi.Tuning) == 6 * 4
+    assert ctypes.sizeof(capi.Tuning) == 9 * 4
 
 
 def test_create_without_device_fails_cleanly():

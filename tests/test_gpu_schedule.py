@@ -87,10 +87,34 @@ def test_schedule_bitwise(gpu_ready, ragged, schedule):
     ([32, 64, 64], [8, 4, 4]),
     ([64, 64, 64], [1, 2, 4]),
     ([16, 32, 64], [16, 8, 4]),
-], ids=["lanes16-32-64-wg64", "lanes32-64-64", "lanes64-all", "default"])
+    ([16, 16, 32], [4, 4, 2], [1, 2, 2]),
+    ([16, 32, 32], [16, 8, 8], [2, 2, 2]),
+    ([16, 16, 64], [4, 8, 2], [1, 2, 2]),
+], ids=["lanes16-32-64-wg64", "lanes32-64-64", "lanes64-all", "default", "two-layers-wg64", "two-layers-all",
+        "two-layers-wide"])
 def test_kernel_shape_bitwise(gpu_ready, ragged, tuning):
+    """Lanes per column, workgroup size and layers per lane (two layers per
+    lane: k_emp_adaptive2) all give the default shape's results."""
     dom, ref = ragged
     out = _run(dom, tuning=tuning)
+    print(tuning, "ms", out["ms"], flush=True)
+    _assert_same(out, ref)
+
+
+@pytest.mark.parametrize("change", ["geometric", "ros2"])
+def test_two_layers_fall_back_where_unsupported(gpu_ready, change):
+    """A two-layers-per-lane shape installed on a handle whose parameters
+    need the one-layer kernels (geometric-mean weights, the linearly
+    implicit stepper) runs those, with the same results as the default
+    shape."""
+    dom = _ragged_series(ncol=60, seed=5)
+    if change == "geometric":
+        dom.params.emp_weights = 1
+    else:
+        dom.params.adaptive_method = 7
+    span = 2 * 3600.0
+    ref = _run(dom, days=span)
+    out = _run(dom, tuning=([16, 16, 32], [4, 4, 2], [1, 2, 2]), days=span)
     _assert_same(out, ref)
 
 
@@ -119,10 +143,12 @@ def test_autotune_restores_state_and_installs_shape(gpu_ready, ragged):
         assert np.array_equal(g.diagnostics(), d0)
         assert g.get_tuning() == best
         for b, lmin in enumerate((16, 32, 64)):
-            assert best["lanes_per_column"][b] >= lmin
+            assert best["layers_per_lane"][b] in (1, 2)
+            assert best["lanes_per_column"][b] >= max(16, lmin // best["layers_per_lane"][b])
             assert best["lanes_per_column"][b] * best["columns_per_group"][b] in (64, 128, 256)
-            valid = table[b][table[b] >= 0.0]
-            assert valid.size >= 3 and np.all(valid > 0.0)
+            for ip in range(2):  # both layers-per-lane choices are timed
+                valid = table[b, ip][table[b, ip] >= 0.0]
+                assert valid.size >= 3 and np.all(valid > 0.0)
         print("autotune", best, "\n", np.round(table, 3), flush=True)
         g.integrate(0.0, SPAN)
         out = dict(x=g.get_state(), f=g.fluxes(), b=g.budget(), d=g.diagnostics())
@@ -140,6 +166,14 @@ def test_schedule_and_tuning_errors(gpu_ready):
             with pytest.raises(capi.SedcolError) as e:
                 g.set_tuning(lanes, cpg)
             assert e.value.code == capi.SEDCOL_EINVAL
+        # layers per lane: 1 or 2, and lanes for half the layers at two
+        for lanes, cpg, lpl in (([16, 32, 64], [4, 2, 1], [1, 3, 1]), ([16, 16, 16], [4, 4, 4], [1, 2, 2])):
+            with pytest.raises(capi.SedcolError) as e:
+                g.set_tuning(lanes, cpg, lpl)
+            assert e.value.code == capi.SEDCOL_EINVAL
+        g.set_tuning([16, 16, 32], [4, 4, 2], [1, 2, 2])
+        assert g.get_tuning() == {"lanes_per_column": [16, 16, 32], "columns_per_group": [4, 4, 2],
+                                  "layers_per_lane": [1, 2, 2]}
         assert g.get_schedule() == dict(window=0, graph_windows=0, check_every=1, concurrent_bins=0)
 
 
