@@ -101,6 +101,25 @@ def test_kernel_shape_bitwise(gpu_ready, ragged, tuning):
     _assert_same(out, ref)
 
 
+def test_two_layers_fast_form_shelf_bitwise(gpu_ready):
+    """The bench's arithmetic form (fast) with two layers per lane on 16
+    lanes (K = 30): shelf-sea columns over two daily forcing records, equal
+    to the default shape and to the oracle bit for bit."""
+    full = cases.shelf_sea(n=40, days=2, seed=3)
+    dom = full.subset(np.flatnonzero(full.mask == 0)[:200])
+    dom.params.arith = 1
+    span = 2 * 86400.0
+    ref = _run(dom, days=span)
+    out = _run(dom, tuning=([16, 16, 32], [16, 4, 8], [1, 2, 2]), days=span)
+    _assert_same(out, ref)
+    orc = O.Oracle(dom, dom.params.si())
+    xo = dom.conc0.copy()
+    spans = cases.record_spans(0.0, span, dom.nrec, dom.t0, dom.dt_rec)
+    assert orc.integrate_series(xo, 0.0, span, spans, nthreads=8) == 0
+    assert np.array_equal(out["x"], xo)
+    assert np.array_equal(out["d"]["accepted"], orc.diag["accepted"])
+
+
 @pytest.mark.parametrize("change", ["geometric", "ros2"])
 def test_two_layers_fall_back_where_unsupported(gpu_ready, change):
     """A two-layers-per-lane shape installed on a handle whose parameters
