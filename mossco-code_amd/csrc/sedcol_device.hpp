@@ -568,8 +568,17 @@ __device__ __forceinline__ void column_rhs(const sedcol_params &p, const Lane &L
     double F;
     if (ARITH != ARITH_FAST) {
       const double D = pt ? Dbio : Ddis;
-      if (top) F = (bc == 1 || bc == 4) ? sbc[NV + n] : bc == 2 ? -((D * (x - sbc[n])) / L.dz) : 0.0;
-      else F = -((D * (x - xup)) / L.dzc);
+      {
+        // one flux expression for the top (bcup = 2, against the bottom
+        // water over dz) and the interior (against the layer above over
+        // dzc), then a select: the same operations as two branches, without
+        // the divergent control flow around every variable
+        const double xo = top ? sbc[n] : xup;
+        const double den = top ? L.dz : L.dzc;
+        const double Fd = -((D * (x - xo)) / den);
+        const double Ft = (bc == 1 || bc == 4) ? sbc[NV + n] : bc == 2 ? Fd : 0.0;
+        F = top ? Ft : Fd;
+      }
       if (ARITH == ARITH_EXT && bc == 4) {
         // diff3d BcUp = 4 (fabm_sediment_driver.F90:791-802): the top flux
         // beyond the first layer's cap is added to the interfaces below,
@@ -599,8 +608,16 @@ __device__ __forceinline__ void column_rhs(const sedcol_params &p, const Lane &L
       cb(n, R[n] + Q, F);
     } else {
       const double G = pt ? L.Gb : L.Gd;
-      if (top) F = bc == 1 ? sbc[NV + n] : bc == 2 ? -(G * (x - sbc[n])) : 0.0;
-      else F = -(G * (x - xup));
+      {
+        const double xo = top ? sbc[n] : xup;
+        const double Fd = -(G * (x - xo));
+        const double Ft = bc == 1 ? sbc[NV + n] : bc == 2 ? Fd : 0.0;
+        F = top ? Ft : Fd;
+      }
+      // the lane exchange runs on every lane (a lane outside the exec mask
+      // reads as 0 to the lane above it), then the bottom value replaces it;
+      // written as a select on lg_dn's result it would only be evaluated
+      // on the lanes that use it
       double Fdn = lg_dn(F);
       if (bottom) Fdn = 0.0;
       const double Q = (F - Fdn) * L.W;
