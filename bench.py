@@ -39,11 +39,11 @@ DAY = 86400.0
 #: HBM bytes per k_emp_adaptive launch of the default workload (300 x 300,
 #: seed 0, fast form, EMP, 5 untimed days then 20 timed days), from
 #: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of that same command over
-#: its 20 timed launches: mean FETCH_SIZE 451 318 KiB x 2.00 + WRITE_SIZE
-#: 272 064 KiB x 1.00, the factors being what the counters read for a plain
+#: its 20 timed launches: mean FETCH_SIZE 458 544 KiB x 2.00 + WRITE_SIZE
+#: 273 073 KiB x 1.00, the factors being what the counters read for a plain
 #: 256 MiB copy (FETCH_SIZE 131 086 KiB, WRITE_SIZE 262 144 KiB).  Reported
 #: for that window only.
-DEFAULT_TRAFFIC_BYTES = (451318 * 2.0 + 272064 * 1.0) * 1024
+DEFAULT_TRAFFIC_BYTES = (458544 * 2.0 + 273073 * 1.0) * 1024
 
 
 def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads, cols_per_thread=32):
