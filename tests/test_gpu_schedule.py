@@ -120,6 +120,26 @@ def test_two_layers_fast_form_shelf_bitwise(gpu_ready):
     assert np.array_equal(out["d"]["accepted"], orc.diag["accepted"])
 
 
+@pytest.mark.parametrize("arith", [0, 1], ids=["reference", "fast"])
+def test_two_layers_layer_count_edges(gpu_ready, arith):
+    """Two layers per lane at the edges of the lane mapping: K = 1 (top and
+    bottom in sub-layer 0 of lane 0), odd K (the last lane's second layer
+    empty), K = 2 LPC (every lane full), in every bin and with the widest
+    lanes; bitwise equal to one layer per lane."""
+    from sedcol import spinup
+
+    rng = np.random.Generator(np.random.Philox(9))
+    Ks = np.array([1, 2, 3, 7, 15, 16, 17, 18, 31, 32, 33, 34, 47, 63, 64, 64])
+    dom = spinup.build(spinup.draw_inputs(rng, Ks.size), Ks)
+    dom.params.arith = arith
+    span = 6 * 3600.0
+    ref = _run(dom, days=span)
+    for tuning in (([16, 16, 32], [4, 8, 4], [2, 2, 2]), ([64, 64, 64], [4, 2, 1], [2, 2, 2])):
+        out = _run(dom, tuning=tuning, days=span)
+        print(tuning, "accepted", int(out["d"]["accepted"].sum()), flush=True)
+        _assert_same(out, ref)
+
+
 @pytest.mark.parametrize("change", ["geometric", "ros2"])
 def test_two_layers_fall_back_where_unsupported(gpu_ready, change):
     """A two-layers-per-lane shape installed on a handle whose parameters
