@@ -9,8 +9,10 @@
 //              + GELU storing the pre-activation, bias only, GELU'
 //              product, plain fp32)
 //   k_loss     squared error, its gradient in bf16, per-block loss sums
-//   k_colsum_part, k_sum_slices  bias gradients (column sums) and the
-//              split-k weight gradients, summed in a fixed order
+//   k_sum_slices  the split-k weight gradients, summed in a fixed order;
+//              the bias gradients (column sums of dZ) come out of the
+//              weight-gradient product as the row of a ones column
+//              appended to its left operand
 //   k_adam     Adam on the fp32 master parameters, writing the bf16 copy
 //   k_gather   minibatch rows of the resident training set, bf16 inputs
 // Layer widths are padded on the device (inputs to 32, outputs to a
@@ -108,7 +110,7 @@ __device__ inline bf16 elem8(const uint4 &w, int j)
 // order of 16-wide steps, as for any tile width.
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, const bf16 *__restrict__ A, int lda,
-                                              const bf16 *__restrict__ B, int ldb, Epi e)
+                                              const bf16 *__restrict__ B, int ldb, Epi e, int ones_row)
 {
   constexpr int LDK = BK + 8;  // row pitch of the tiles: 144 bytes, 16-byte aligned rows
   __shared__ __attribute__((aligned(16))) bf16 sA[2][BM][LDK];
@@ -132,7 +134,22 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, c
       const int idx = tid * 8 + 2048 * h;
       if (TA) {  // A stored k-major: m fastest
         const int m = idx % BM, k = idx / BM;
-        ra[h] = load8(A + (size_t)(k0 + k) * lda, bm + m, M, k0 + k < kend);
+        // row ones_row of op(A) is all ones (read nothing there): the
+        // product's row ones_row is then the column sums of op(B)
+        const int lim = ones_row >= 0 ? ones_row : M;
+        ra[h] = load8(A + (size_t)(k0 + k) * lda, bm + m, lim, k0 + k < kend);
+        const int j1 = ones_row - (bm + m);
+        if (j1 >= 0 && j1 < 8 && k0 + k < kend) {
+          const unsigned one = 0x3F80u;  // bf16 1.0
+          if (j1 == 0) ra[h].x = (ra[h].x & 0xffff0000u) | one;
+          else if (j1 == 1) ra[h].x = (ra[h].x & 0xffffu) | (one << 16);
+          else if (j1 == 2) ra[h].y = (ra[h].y & 0xffff0000u) | one;
+          else if (j1 == 3) ra[h].y = (ra[h].y & 0xffffu) | (one << 16);
+          else if (j1 == 4) ra[h].z = (ra[h].z & 0xffff0000u) | one;
+          else if (j1 == 5) ra[h].z = (ra[h].z & 0xffffu) | (one << 16);
+          else if (j1 == 6) ra[h].w = (ra[h].w & 0xffff0000u) | one;
+          else ra[h].w = (ra[h].w & 0xffffu) | (one << 16);
+        }
       } else {
         const int m = idx / BK, k = idx % BK;
         ra[h] = load8(A + (size_t)(bm + m) * lda, k0 + k, kend, bm + m < M);
@@ -216,23 +233,6 @@ __global__ void k_sum_slices(const float *part, int S, long long stride, long lo
     for (int sl = 0; sl < S; ++sl) a += part[sl * stride + i];
     out[i] = a;
   }
-}
-
-// column sums of dZ (b x n, leading dimension ld) over row chunks of
-// `rows` rows: part[chunk n_p + c]; block = 64 columns x 4 row lanes
-__global__ __launch_bounds__(256) void k_colsum_part(const bf16 *dZ, int b, int n, int ld, int rows, int n_p,
-                                                     float *part)
-{
-  __shared__ float red[4][64];
-  const int c = blockIdx.x * 64 + (threadIdx.x & 63), lr = threadIdx.x >> 6;
-  const int r0 = blockIdx.y * rows, r1 = min(b, r0 + rows);
-  float s = 0.0f;
-  if (c < n)
-    for (int i = r0 + lr; i < r1; i += 4) s += from_bf16(dZ[(size_t)i * ld + c]);
-  red[lr][threadIdx.x & 63] = s;
-  __syncthreads();
-  if (lr == 0 && c < n)
-    part[(size_t)blockIdx.y * n_p + c] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 // rows of the fp32 sets (or of X itself when rows is null) into padded
@@ -373,7 +373,6 @@ struct sedsurr_handle {
   bf16 *dz0 = nullptr, *dz1 = nullptr;  // max_batch x max(n_hidden, nout_p)
   float *part = nullptr;                // loss partial sums
   float *kpart = nullptr;               // split-k partial weight gradients
-  float *cpart = nullptr;               // partial column sums
   float *loss = nullptr;                // per-step losses of a train call
   int loss_cap = 0;
   float *X = nullptr, *Y = nullptr;     // training set
@@ -393,17 +392,17 @@ namespace {
 constexpr int kLossBlocks = 256;
 
 constexpr int kMaxSplit = 8;     // k slices of a weight-gradient product
-constexpr int kColsumRows = 128;  // rows per partial column sum
 
 template <bool TA, bool TB, int EPI>
 void gemm(sedsurr_handle *h, int M, int N, int K, const bf16 *A, int lda, const bf16 *B, int ldb, ss::Epi e,
-          int *launches, int split = 1)
+          int *launches, int split = 1, int ones_row = -1)
 {
   const int kchunk = (K + split - 1) / split;
   const int kc = (kchunk + ss::BK - 1) / ss::BK * ss::BK;
   const int nz = (K + kc - 1) / kc;
   dim3 grid((N + ss::BN - 1) / ss::BN, (M + ss::BM - 1) / ss::BM, nz > 0 ? nz : 1);
-  hipLaunchKernelGGL((ss::k_gemm<TA, TB, EPI>), grid, dim3(256), 0, h->st, M, N, K, kc, A, lda, B, ldb, e);
+  hipLaunchKernelGGL((ss::k_gemm<TA, TB, EPI>), grid, dim3(256), 0, h->st, M, N, K, kc, A, lda, B, ldb, e,
+                     ones_row);
   ++*launches;
 }
 
@@ -462,14 +461,17 @@ void loss_backward(sedsurr_handle *h, int b, float *loss_dev, int *launches)
   for (int l = L - 1; l >= 0; --l) {
     const bf16 *Hin = l == 0 ? h->xb : h->hs[l - 1];
     const int win = in_width(h, l), wout = h->fop[l];
-    // dW_l = H_in^T dZ_l, split over the batch into slices summed in order
-    const int split = split_for(win, wout, b);
-    const long long welems = (long long)win * wout;
+    // [dW_l; db_l] = [H_in, 1]^T dZ_l in one product: the bias gradient
+    // (the column sums of dZ_l) is the row of a ones column appended to
+    // H_in, and b_l follows W_l in the parameter vector, so that row lands
+    // on it; split over the batch into slices summed in order
+    const int split = split_for(win + 1, wout, b);
+    const long long welems = (long long)(win + 1) * wout;
     ss::Epi e{};
     e.out_f = split > 1 ? h->kpart : h->g + h->woff[l];
     e.ldo = wout;
     e.zstride = welems;
-    gemm<true, false, ss::EPI_F32>(h, win, wout, b, Hin, win, dz, wout, e, launches, split);
+    gemm<true, false, ss::EPI_F32>(h, win + 1, wout, b, Hin, win, dz, wout, e, launches, split, win);
     if (split > 1) {
       const int kc = ((b + split - 1) / split + ss::BK - 1) / ss::BK * ss::BK;
       const int nz = (b + kc - 1) / kc;
@@ -477,13 +479,6 @@ void loss_backward(sedsurr_handle *h, int b, float *loss_dev, int *launches)
                          h->g + h->woff[l]);
       ++*launches;
     }
-    // db_l = column sums of dZ_l, by row chunks, summed in order
-    const int chunks = (b + kColsumRows - 1) / kColsumRows;
-    hipLaunchKernelGGL(ss::k_colsum_part, dim3((wout + 63) / 64, chunks), dim3(256), 0, h->st, dz, b, wout, wout,
-                       kColsumRows, wout, h->cpart);
-    hipLaunchKernelGGL(ss::k_sum_slices, dim3((wout + 255) / 256), dim3(256), 0, h->st, h->cpart, chunks,
-                       (long long)wout, (long long)wout, h->g + h->boff[l]);
-    *launches += 2;
     if (l > 0) {
       // dZ_{l-1} = GELU'(Z_{l-1}) * (dZ_l W_l^T)
       ss::Epi d{};
@@ -595,9 +590,8 @@ int sedsurr_create(const sedsurr_shape *shape, sedsurr_handle **out)
   HIPOK_C(hipMalloc(&h->dz1, sizeof(bf16) * B * wmax));
   HIPOK_C(hipMalloc(&h->part, sizeof(float) * (kLossBlocks + 1)));  // + the loss of sedsurr_loss_grad
   long long wmax_el = 0;
-  for (int l = 0; l < h->nl; ++l) wmax_el = std::max(wmax_el, (long long)h->fip[l] * h->fop[l]);
+  for (int l = 0; l < h->nl; ++l) wmax_el = std::max(wmax_el, (long long)(h->fip[l] + 1) * h->fop[l]);
   HIPOK_C(hipMalloc(&h->kpart, sizeof(float) * kMaxSplit * wmax_el));
-  HIPOK_C(hipMalloc(&h->cpart, sizeof(float) * ((B + kColsumRows - 1) / kColsumRows) * wmax));
   HIPOK_C(hipMalloc(&h->xin, sizeof(float) * B * s.n_in));
   HIPOK_C(hipMalloc(&h->yin, sizeof(float) * B * s.n_out));
 #undef HIPOK_C
@@ -611,7 +605,7 @@ int sedsurr_destroy(sedsurr_handle *h)
   const DeviceScope dscope(h->device);
   if (h->st) (void)hipStreamSynchronize(h->st);
   std::vector<void *> ptrs = {h->w, h->g, h->m, h->v, h->wb, h->xb, h->yh, h->yb, h->dz0, h->dz1,
-                              h->part, h->loss, h->X, h->Y, h->rows, h->xin, h->yin, h->kpart, h->cpart};
+                              h->part, h->loss, h->X, h->Y, h->rows, h->xin, h->yin, h->kpart};
   for (auto *q : h->hs) ptrs.push_back(q);
   for (auto *q : h->zs) ptrs.push_back(q);
   for (void *q : ptrs)
