@@ -231,8 +231,10 @@ def spin(dom: cases.Domain, years: float, tol: float, chunk_days: float = 10.0, 
 
 
 def generate(n: int, seed: int, years: float, tol: float = 1.0e-8, nlayers=K_SURR, tails: float = 0.0,
-             chunk_days: float = 10.0):
-    """Cold spin-up of n drawn columns (nlayers: int or per-column array).
+             chunk_days: float = 10.0, method: int = 0, dt_max: float | None = None):
+    """Cold spin-up of n drawn columns (nlayers: int or per-column array)
+    with adaptive stepper `method` (0 or 6 adaptive EMP, 7 linearly
+    implicit) and steps up to dt_max (None: the parameter default).
     Returns the inputs, the final packed state, per-column residual and
     convergence time, top fluxes at the final state and the wall time."""
     rng = np.random.Generator(np.random.Philox(seed))
@@ -240,6 +242,9 @@ def generate(n: int, seed: int, years: float, tol: float = 1.0e-8, nlayers=K_SUR
     if np.isscalar(nlayers) and nlayers == 0:
         nlayers = rng.integers(20, 41, size=n)
     dom = build(s, nlayers)
+    dom.params.adaptive_method = method
+    if dt_max is not None:
+        dom.params.emp_dt_max = dt_max
     out = spin(dom, years, tol, chunk_days=chunk_days, log=lambda m: print(m, flush=True))
     out["inputs"] = s
     out["nlayers"] = np.asarray(dom.nlayers)
@@ -258,10 +263,13 @@ def _main(argv=None):
     ap.add_argument("--tol", type=float, default=1.0e-8)
     ap.add_argument("--ragged", action="store_true", help="layers U{20..40} instead of 30")
     ap.add_argument("--tails", type=float, default=0.0)
+    ap.add_argument("--method", type=int, default=0, help="adaptive stepper: 0 EMP, 7 linearly implicit ROS2")
+    ap.add_argument("--dt-max", type=float, default=None, help="largest step (s); default: the parameter default")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     t = time.perf_counter()
-    dom, o = generate(a.n, a.seed, a.years, a.tol, nlayers=0 if a.ragged else K_SURR, tails=a.tails)
+    dom, o = generate(a.n, a.seed, a.years, a.tol, nlayers=0 if a.ragged else K_SURR, tails=a.tails,
+                      method=a.method, dt_max=a.dt_max)
     s = o["inputs"]
     save = {"X": as_matrix(s), "tail": s["tail"], "nlayers": o["nlayers"], "resid": o["resid"],
             "t_conv": o["t_conv"], "flux": o["flux"], "wall_s": o["wall_s"]}
