@@ -281,7 +281,7 @@ def predict(model: Model, X, net: Network | None = None):
 
 
 def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_tol=1.0e-6, flux_rtol=0.01,
-             tails=0.05, adaptive_method=7, dt_max=86400.0, chunk_days=1.0, log=None):
+             tails=0.05, adaptive_method=7, dt_max=86400.0, chunk_days=1.0, log=None, dump=None):
     """Surrogate warm start against the plain cold spin-up on n drawn columns
     (layers U{20..40}, a fraction `tails` of the inputs outside the training
     ranges).  Every integration uses the same stepper (adaptive_method,
@@ -364,6 +364,13 @@ def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_
             "flux_within_1pct_residual_only": float(np.mean(fl_ro[v][ro] <= 0.01)) if ro.any() else None,
         }
     tail = s["tail"]
+    if dump:
+        # per-column record: cold attempts and final residual, warm
+        # retirement time (-1: fallback), residual-only retirement time
+        np.savez_compressed(dump, cold_attempts=cold["attempts"], cold_resid=cold["resid"],
+                            warm_t_conv=warm["t_conv"], warm_attempts=warm["attempts"],
+                            resonly_t_conv=res_only["t_conv"], tail=tail, nlayers=np.asarray(K),
+                            flux_err_warm=fl_err, flux_err_resonly=fl_ro)
     return {
         "columns": n, "tail_fraction": float(tail.mean()), "years": years, "warm_days": warm_days,
         "adaptive_method": adaptive_method, "dt_max_s": dt_max, "res_tol": res_tol, "flux_rtol": flux_rtol,
@@ -410,6 +417,7 @@ def _main(argv=None):
     e.add_argument("--warm-days", type=float, default=30.0)
     e.add_argument("--method", type=int, default=7, help="adaptive stepper: 6 EMP, 7 linearly implicit ROS2")
     e.add_argument("--dt-max", type=float, default=86400.0)
+    e.add_argument("--dump", default=None, help="npz file for the per-column record")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "train":
@@ -447,7 +455,7 @@ def _main(argv=None):
     else:
         model = Model.load(a.model)
         r = evaluate(model, a.n, a.seed, years=a.years, warm_days=a.warm_days, adaptive_method=a.method,
-                     dt_max=a.dt_max, log=log)
+                     dt_max=a.dt_max, log=log, dump=a.dump)
         print(json.dumps(r), flush=True)
 
 
