@@ -172,6 +172,7 @@ def ragged_bench(args):
     if t_w > 0:
         g.integrate(0.0, t_w)
     ck = g.save_checkpoint()
+    state_w = g.get_state()
     schedules = [
         ("persistent", dict(window=0, graph_windows=0, check_every=1, concurrent_bins=0)),
         ("persistent-concurrent-bins", dict(window=0, graph_windows=0, check_every=1, concurrent_bins=1)),
@@ -241,6 +242,16 @@ def ragged_bench(args):
         "rows": allrows,
         "input_generation_s": gen_s,
     }
+    if not args.no_cpu:
+        # the oracle on a sample of the same ragged columns, same span from
+        # the same warmed state, on up to 16 host cores and on one
+        nthreads = min(16, os.cpu_count() or 1)
+        ndays = max(1, int(round(span / DAY)))
+        # a 10 240-column subset (one day is a few seconds of oracle work)
+        out["cpu_baseline"] = cpu_baseline(dom, state_w, args.warmup, ndays, args.cpu_seconds, nthreads,
+                                           cols_per_thread=10240 // nthreads)
+        one = cpu_baseline(dom, state_w, args.warmup, ndays, args.cpu_seconds / 2, 1, cols_per_thread=600)
+        out["cpu_baseline"]["one_core"] = {k: one[k] for k in ("value", "unit", "cores", "sample")}
     print(json.dumps(out), flush=True)
     return 0 if same else 1
 
