@@ -129,6 +129,23 @@ def denit_diagnostic(n2_release):
 
 
 # ---------------------------------------------------------------------------
+# export fields
+# ---------------------------------------------------------------------------
+
+def upward_flux_fields(flux_top, ni: int, nj: int, names=STATE_NAMES) -> dict:
+    """Export side of the top fluxes (fabm_sediment_component.F90:1776-1819):
+    {<name>_upward_flux_at_soil_surface: (I, J) array} from the library's
+    top fluxes (nvar, ncol), column c = i + I j.  The library's fluxes are
+    positive downward, the export positive upward: the field is -flux
+    (:1798, 1812)."""
+    f = np.asarray(flux_top, dtype=np.float64)
+    if f.shape != (len(names), ni * nj):
+        raise ValueError(f"flux_top must be ({len(names)}, {ni * nj})")
+    return {f"{n}_upward_flux_at_soil_surface": np.ascontiguousarray((0.0 - f[v]).reshape(nj, ni).T)
+            for v, n in enumerate(names)}
+
+
+# ---------------------------------------------------------------------------
 # restart fields
 # ---------------------------------------------------------------------------
 

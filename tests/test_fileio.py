@@ -14,7 +14,7 @@ import pytest
 
 from sedcol import cases, grid
 from sedcol.fileio import (DENIT_NAME, STATE_NAMES, OutputDat, apply_restart_fields, fortran_e, load_restart,
-                           restart_fields, save_restart)
+                           restart_fields, save_restart, upward_flux_fields)
 
 
 @pytest.mark.parametrize("x,w,d,e,want", [
@@ -115,3 +115,20 @@ def test_update_porosity_from_surface():
         assert fc[sl][0] == cap[0] and fc[sl][1] == cap[1]
     sl = slice(off[1], off[2])
     assert np.all(phi[sl] == 1.0) and np.all(phiI[sl] == 1.0)
+
+
+def test_upward_flux_export_sign_and_layout():
+    """*_upward_flux_at_soil_surface = -flux (fabm_sediment_component.F90:
+    1798,1812) on the (I, J) grid, column c = i + I j."""
+    ni, nj = 3, 2
+    flux = np.arange(11 * ni * nj, dtype=np.float64).reshape(11, ni * nj) - 7.0
+    f = upward_flux_fields(flux, ni, nj)
+    assert set(f) == {n + "_upward_flux_at_soil_surface" for n in STATE_NAMES}
+    for v, n in enumerate(STATE_NAMES):
+        a = f[n + "_upward_flux_at_soil_surface"]
+        assert a.shape == (ni, nj)
+        for i in range(ni):
+            for j in range(nj):
+                assert a[i, j] == -flux[v, i + ni * j]
+    with pytest.raises(ValueError):
+        upward_flux_fields(flux[:, :5], ni, nj)
