@@ -128,12 +128,17 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, c
   // this thread's share of a 64 x 64 tile of each operand: eight
   // consecutive elements at idx = 8 tid + 2048 h, h = 0, 1
   uint4 ra[2], rb[2];
+  // Operands stored k-major (TA for A, !TB for B) are transposed into the
+  // k-contiguous LDS tiles: a thread loads the same eight m (or n) at two
+  // consecutive k (k even), so every tile element pair (m, k..k+1) is one
+  // 4-byte LDS write.
+  const int tm = (tid % 8) * 8, tk = 2 * (tid / 8);
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int idx = tid * 8 + 2048 * h;
       if (TA) {  // A stored k-major: m fastest
-        const int m = idx % BM, k = idx / BM;
+        const int m = tm, k = tk + h;
         // row ones_row of op(A) is all ones (read nothing there): the
         // product's row ones_row is then the column sums of op(B)
         const int lim = ones_row >= 0 ? ones_row : M;
@@ -158,31 +163,36 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, c
         const int n = idx / BK, kb = idx % BK;
         rb[h] = load8(B + (size_t)(bn + n) * ldb, k0 + kb, kend, bn + n < N);
       } else {
-        const int kb = idx / BN, n = idx % BN;
+        const int kb = tk + h, n = tm;
         rb[h] = load8(B + (size_t)(k0 + kb) * ldb, bn + n, N, k0 + kb < kend);
       }
     }
   };
+  auto pair = [](const uint4 &lo, const uint4 &hi, int j) {
+    return (unsigned)elem8(lo, j) | ((unsigned)elem8(hi, j) << 16);
+  };
   auto stage = [&](int buf) {
+    if (TA) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int idx = tid * 8 + 2048 * h;
-      if (TA) {
-        const int m = idx % BM, k = idx / BM;
+      for (int j = 0; j < 8; ++j) *reinterpret_cast<unsigned *>(&sA[buf][tm + j][tk]) = pair(ra[0], ra[1], j);
+    } else {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) sA[buf][m + j][k] = elem8(ra[h], j);
-      } else {
+      for (int h = 0; h < 2; ++h) {
+        const int idx = tid * 8 + 2048 * h;
         const int m = idx / BK, k = idx % BK;
         *reinterpret_cast<uint4 *>(&sA[buf][m][k]) = ra[h];
       }
-      if (TB) {
+    }
+    if (TB) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int idx = tid * 8 + 2048 * h;
         const int n = idx / BK, kb = idx % BK;
         *reinterpret_cast<uint4 *>(&sB[buf][n][kb]) = rb[h];
-      } else {
-        const int kb = idx / BN, n = idx % BN;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) sB[buf][n + j][kb] = elem8(rb[h], j);
       }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) *reinterpret_cast<unsigned *>(&sB[buf][tm + j][tk]) = pair(rb[0], rb[1], j);
     }
   };
 
