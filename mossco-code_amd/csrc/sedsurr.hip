@@ -303,11 +303,30 @@ __global__ __launch_bounds__(256) void k_sum(const float *part, int n, float sca
 }
 
 // Adam on the flat parameter vector; wb receives the bf16 copy
+// Split-k slices of the weight gradients that Adam sums itself: layer l's
+// parameters [woff, woff + welems) take their gradient from nz slices at
+// part + pofs (slice z at + z welems) when nz > 1, else from g
+constexpr int kMaxLayers = 17;  // n_layers <= 16 hidden layers + the output layer
+struct Slices {
+  int nl;
+  long long woff[kMaxLayers], welems[kMaxLayers], pofs[kMaxLayers];
+  int nz[kMaxLayers];
+};
+
 __global__ void k_adam(float *w, const float *g, float *m, float *v, bf16 *wb, long long n, float lr, float b1,
-                       float b2, float eps, float bc1, float bc2)
+                       float b2, float eps, float bc1, float bc2, const float *part, Slices sl)
 {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const float gi = g[i];
+    float gi = g[i];
+    for (int l = 0; l < sl.nl; ++l) {
+      const long long j = i - sl.woff[l];
+      if (j >= 0 && j < sl.welems[l] && sl.nz[l] > 1) {
+        // the slices in order, as k_sum_slices adds them
+        float a = 0.0f;
+        for (int z = 0; z < sl.nz[l]; ++z) a += part[sl.pofs[l] + z * sl.welems[l] + j];
+        gi = a;
+      }
+    }
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
@@ -383,6 +402,9 @@ struct sedsurr_handle {
   bf16 *dz0 = nullptr, *dz1 = nullptr;  // max_batch x max(n_hidden, nout_p)
   float *part = nullptr;                // loss partial sums
   float *kpart = nullptr;               // split-k partial weight gradients
+  float *kpart_all = nullptr;           // the same, one region per layer (training: summed by Adam)
+  std::vector<long long> kpofs;         // offset of layer l's region in kpart_all
+  ss::Slices slices{};                  // the last training step's deferred slices
   float *loss = nullptr;                // per-step losses of a train call
   int loss_cap = 0;
   float *X = nullptr, *Y = nullptr;     // training set
@@ -457,8 +479,9 @@ void forward(sedsurr_handle *h, int b, int *launches, float *out = nullptr)
 }
 
 // loss of the forward output against h->yb, gradient into h->g; loss value
-// to *loss_dev
-void loss_backward(sedsurr_handle *h, int b, float *loss_dev, int *launches)
+// to *loss_dev.  defer: split weight gradients stay as slices in
+// h->kpart_all (described in h->slices) for k_adam to sum
+void loss_backward(sedsurr_handle *h, int b, float *loss_dev, int *launches, bool defer = false)
 {
   const int L = h->nl;
   const float scale = 2.0f / ((float)b * (float)h->s.n_out);
@@ -477,14 +500,19 @@ void loss_backward(sedsurr_handle *h, int b, float *loss_dev, int *launches)
     // on it; split over the batch into slices summed in order
     const int split = split_for(win + 1, wout, b);
     const long long welems = (long long)(win + 1) * wout;
+    const int kc = ((b + split - 1) / split + ss::BK - 1) / ss::BK * ss::BK;
+    const int nz = (b + kc - 1) / kc;
+    float *part = defer ? h->kpart_all + h->kpofs[l] : h->kpart;
     ss::Epi e{};
-    e.out_f = split > 1 ? h->kpart : h->g + h->woff[l];
+    e.out_f = nz > 1 ? part : h->g + h->woff[l];
     e.ldo = wout;
     e.zstride = welems;
     gemm<true, false, ss::EPI_F32>(h, win + 1, wout, b, Hin, win, dz, wout, e, launches, split, win);
-    if (split > 1) {
-      const int kc = ((b + split - 1) / split + ss::BK - 1) / ss::BK * ss::BK;
-      const int nz = (b + kc - 1) / kc;
+    h->slices.woff[l] = h->woff[l];
+    h->slices.welems[l] = welems;
+    h->slices.pofs[l] = h->kpofs[l];
+    h->slices.nz[l] = defer ? nz : 1;
+    if (nz > 1 && !defer) {
       hipLaunchKernelGGL(ss::k_sum_slices, dim3(512), dim3(256), 0, h->st, h->kpart, nz, welems, welems,
                          h->g + h->woff[l]);
       ++*launches;
@@ -602,6 +630,14 @@ int sedsurr_create(const sedsurr_shape *shape, sedsurr_handle **out)
   long long wmax_el = 0;
   for (int l = 0; l < h->nl; ++l) wmax_el = std::max(wmax_el, (long long)(h->fip[l] + 1) * h->fop[l]);
   HIPOK_C(hipMalloc(&h->kpart, sizeof(float) * kMaxSplit * wmax_el));
+  if (h->nl > ss::kMaxLayers) return cleanup(fail(SEDCOL_EINVAL, "too many layers"));
+  long long kall = 0;
+  for (int l = 0; l < h->nl; ++l) {
+    h->kpofs.push_back(kall);
+    kall += (long long)kMaxSplit * (h->fip[l] + 1) * h->fop[l];
+  }
+  HIPOK_C(hipMalloc(&h->kpart_all, sizeof(float) * kall));
+  h->slices.nl = h->nl;
   HIPOK_C(hipMalloc(&h->xin, sizeof(float) * B * s.n_in));
   HIPOK_C(hipMalloc(&h->yin, sizeof(float) * B * s.n_out));
 #undef HIPOK_C
@@ -615,7 +651,7 @@ int sedsurr_destroy(sedsurr_handle *h)
   const DeviceScope dscope(h->device);
   if (h->st) (void)hipStreamSynchronize(h->st);
   std::vector<void *> ptrs = {h->w, h->g, h->m, h->v, h->wb, h->xb, h->yh, h->yb, h->dz0, h->dz1,
-                              h->part, h->loss, h->X, h->Y, h->rows, h->xin, h->yin, h->kpart};
+                              h->part, h->loss, h->X, h->Y, h->rows, h->xin, h->yin, h->kpart, h->kpart_all};
   for (auto *q : h->hs) ptrs.push_back(q);
   for (auto *q : h->zs) ptrs.push_back(q);
   for (void *q : ptrs)
@@ -719,12 +755,12 @@ int sedsurr_train(sedsurr_handle *h, int32_t steps, int32_t batch, const int32_t
                        h->rows + (long long)s * batch, batch, h->kin, h->nout_p, h->xb, h->yb);
     ++launches;
     forward(h, batch, &launches);
-    loss_backward(h, batch, h->loss + s, &launches);
+    loss_backward(h, batch, h->loss + s, &launches, true);
     ++h->step;
     const float bc1 = 1.0f - std::pow(adam->beta1, (float)h->step);
     const float bc2 = 1.0f - std::pow(adam->beta2, (float)h->step);
     hipLaunchKernelGGL(ss::k_adam, dim3(1024), dim3(256), 0, h->st, h->w, h->g, h->m, h->v, h->wb, h->np_pad,
-                       adam->lr, adam->beta1, adam->beta2, adam->eps, bc1, bc2);
+                       adam->lr, adam->beta1, adam->beta2, adam->eps, bc1, bc2, h->kpart_all, h->slices);
     ++launches;
   }
   HIPOK(hipGetLastError());
