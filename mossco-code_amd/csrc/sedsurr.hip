@@ -9,7 +9,8 @@
 //              + GELU storing the pre-activation, bias only, GELU'
 //              product, plain fp32)
 //   k_loss     squared error, its gradient in bf16, per-block loss sums
-//   k_sum_slices  the split-k weight gradients, summed in a fixed order;
+//   k_sum_slices  the split-k weight gradients, summed in a fixed order
+//              (for the gradient query; in training k_adam sums them);
 //              the bias gradients (column sums of dZ) come out of the
 //              weight-gradient product as the row of a ones column
 //              appended to its left operand
