@@ -152,6 +152,11 @@ module sedcol_hip
       type(c_ptr), value :: h
       real(c_double), intent(out) :: v
     end function
+    !> clears last_min_dt and the per-column step counters
+    integer(c_int) function sedcol_reset_diagnostics(h) bind(C, name='sedcol_reset_diagnostics')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
     !> per-column spin-up (replaces the presimulation of
     !> fabm_sediment_component.F90:557-632); cfloor(nvar); flux_ref and
     !> flux_floor may be c_null_ptr (residual criterion only)
@@ -177,7 +182,8 @@ module sedcol_hip
   public :: sedcol_create, sedcol_destroy, sedcol_set_params, sedcol_set_geometry, sedcol_set_forcing, &
     sedcol_set_forcing_series, sedcol_set_state, sedcol_get_state, sedcol_rhs, sedcol_step, sedcol_run, &
     sedcol_integrate, sedcol_get_fluxes, sedcol_get_budget, sedcol_get_last_min_dt, sedcol_spinup, &
-    sedcol_rates, sedcol_error_message
+    sedcol_rates, sedcol_reset_diagnostics, sedcol_set_bioturbation_zw, sedcol_set_flux_cap, &
+    sedcol_assemble_forcing, sedcol_error_message
 
   !> GPU-backed counterpart of type_sed + type_rhs_driver
   type, public :: type_sed_hip

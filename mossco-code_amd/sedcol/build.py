@@ -93,8 +93,9 @@ def build_fortran(lib: Path | None = None) -> Path | None:
     _run([FC, "-O2", "-ffp-contract=off", "-c", FSRC / "sedcol_hip.F90", "-o", bdir / "sedcol_hip.o"], cwd=bdir)
     exe = FSRC / "fabmsed_hip"
     rel = os.path.relpath(lib.parent, FSRC)
-    _run([FC, "-O2", "-ffp-contract=off", "-I", bdir, FSRC / "fabmsed_hip.F90", bdir / "sedcol_hip.o",
-          "-o", exe, f"-L{lib.parent}", "-lsedcol", f"-Wl,-rpath,$ORIGIN/{rel}"], cwd=bdir)
+    for prog in ("fabmsed_hip", "fabmsed1d_hip"):
+        _run([FC, "-O2", "-ffp-contract=off", "-I", bdir, FSRC / f"{prog}.F90", bdir / "sedcol_hip.o",
+              "-o", FSRC / prog, f"-L{lib.parent}", "-lsedcol", f"-Wl,-rpath,$ORIGIN/{rel}"], cwd=bdir)
     return exe
 
 
