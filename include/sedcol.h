@@ -231,6 +231,23 @@ int sedcol_set_forcing_series(sedcol_handle *h, int32_t nrec, double t0, double 
 int sedcol_set_state(sedcol_handle *h, const double *conc);
 int sedcol_get_state(sedcol_handle *h, double *conc);
 
+/* Warm start on the device (no reference counterpart: it replaces the state
+ * that the presimulation of fabm_sediment_component.F90:557-632 would
+ * produce, with profiles predicted by a model such as the sedsurr network,
+ * without a round trip through host memory).  prof: device memory of the
+ * handle's device, one row per column (stride ld >= nvar*nz floats); row c
+ * holds u = prof[c*ld + n*nz + j] for variable n at node j, and the profile
+ * is y = u*scale[n*nz+j] + offset[n*nz+j], the log10 of (concentration +
+ * eps[n]).  The nodes are the layer centres of an nz-layer geometric grid
+ * (top layer dz1 m, constant growth) of the column's total depth: depth[c]
+ * (m), or the column's own sum of dz when depth is NULL.  Every layer of a
+ * wet column gets max(10^y(z) - eps[n], 0) at its own centre z, with y
+ * linear in depth between nodes and constant beyond the end nodes (NaN
+ * stays NaN).  Land columns keep their state.  2 <= nz <= 64; scale,
+ * offset (nvar*nz), eps (nvar) and depth (ncol) are host memory. */
+int sedcol_set_state_profiles(sedcol_handle *h, const float *prof, int64_t ld, int32_t nz, const double *scale,
+                              const double *offset, const double *eps, double dz1, const double *depth);
+
 /* --- hot path ----------------------------------------------------------- */
 
 /* get_rhs (fabm_sediment_driver.F90:575-717): rhs of conc under the current

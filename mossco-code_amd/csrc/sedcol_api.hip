@@ -95,6 +95,7 @@ struct sedcol_handle {
   double *d_scr = nullptr;  // NV*ncell scratch (rhs in/out, adaptive Euler rates)
   double *d_scr2 = nullptr; // NV*ncell scratch
   double *d_small = nullptr;  // NV*ncol + ncol scratch
+  double *d_warm = nullptr;   // scale, offset (NV*kWarmMaxNodes each) and eps (NV) of a warm start
   double *d_noalloc = nullptr;  // this device's sedcol_rhs_noalloc workspace (not owned)
   int *d_flag = nullptr;     // [0] error flags, [1..3] work-queue counters per bin, [4] windowed columns pending
   int *h_flag = nullptr;     // pinned host copy of d_flag[0..4]
@@ -640,7 +641,7 @@ int sedcol_destroy(sedcol_handle *h)
                   h->d_bfac, h->d_biot, h->d_temp, h->d_cbw, h->d_fup, h->d_conc, h->d_cellacc, h->d_colacc,
                   h->d_flux, h->d_diag, h->d_scr, h->d_scr2, h->d_small, h->d_flag, h->d_work,
                   h->d_devview, h->d_params, h->d_wrec, h->d_wrem, h->d_watt, h->d_ctl, h->d_fcap,
-                  h->d_ze1, h->d_ze2, h->d_zcum};
+                  h->d_ze1, h->d_ze2, h->d_zcum, h->d_warm};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (h->h_flag) (void)hipHostFree(h->h_flag);
@@ -893,6 +894,41 @@ int sedcol_set_state(sedcol_handle *h, const double *conc)
   std::vector<double> buf;
   pack_state(h, conc, buf);
   HIPOK(hipMemcpyAsync(h->d_conc, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return SEDCOL_OK;
+}
+
+int sedcol_set_state_profiles(sedcol_handle *h, const float *prof, int64_t ld, int32_t nz, const double *scale,
+                              const double *offset, const double *eps, double dz1, const double *depth)
+{
+  if (!h) return fail(SEDCOL_EINVAL, "null handle");
+  const DeviceScope dscope(h->device);
+  if (!h->have_geom) return fail(SEDCOL_EINVAL, "sedcol_set_geometry has not been called");
+  if (!prof || !scale || !offset || !eps) return fail(SEDCOL_EINVAL, "null argument");
+  if (nz < 2 || nz > sc::kWarmMaxNodes) return fail(SEDCOL_EINVAL, "nz must be 2..64");
+  if (ld < (int64_t)NV * nz) return fail(SEDCOL_EINVAL, "ld must be at least nvar*nz");
+  if (!(dz1 > 0.0) || !std::isfinite(dz1)) return fail(SEDCOL_EINVAL, "dz1 must be positive and finite");
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, prof) != hipSuccess || at.type != hipMemoryTypeDevice || at.device != h->device) {
+    (void)hipGetLastError();
+    return fail(SEDCOL_EINVAL, "prof must be device memory of the handle's device");
+  }
+  if (depth)
+    for (int c = 0; c < h->ncol; ++c)
+      if (!h->mask[c] && !(depth[c] > 0.0 && std::isfinite(depth[c])))
+        return fail(SEDCOL_EINVAL, "depth must be positive and finite for every wet column");
+  if (!h->d_warm) HIPOK(hipMalloc(&h->d_warm, sizeof(double) * (2 * NV * sc::kWarmMaxNodes + NV)));
+  const size_t nv = (size_t)NV * nz;
+  HIPOK(hipMemcpyAsync(h->d_warm, scale, sizeof(double) * nv, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_warm + nv, offset, sizeof(double) * nv, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(h->d_warm + 2 * nv, eps, sizeof(double) * NV, hipMemcpyHostToDevice, h->stream));
+  if (depth) HIPOK(hipMemcpyAsync(h->d_small, depth, sizeof(double) * h->ncol, hipMemcpyHostToDevice, h->stream));
+  sc::WarmProfiles w{prof, (long long)ld, nz, dz1, h->d_warm, h->d_warm + nv, h->d_warm + 2 * nv,
+                     depth ? h->d_small : nullptr};
+  const Dev d = h->dev();
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty()) sc::launch_warm_profiles(kLpc[b], h->stream, d, h->d_list[b], (int)h->bins[b].size(), w);
+  HIPOK(hipGetLastError());
   HIPOK(hipStreamSynchronize(h->stream));
   return SEDCOL_OK;
 }

@@ -1788,6 +1788,91 @@ void launch_emp_begin(hipStream_t st, const int *list, int n, int r0, int *wrec,
   hipLaunchKernelGGL(k_emp_begin, dim3(blocks), dim3(256), 0, st, list, n, r0, wrec, wrem, watt);
 }
 
+// ---------------------------------------------------------------------------
+// warm start from log10 profiles on a geometric node grid
+// ---------------------------------------------------------------------------
+
+// ratio r of an n-layer geometric grid, dz1 (r^n - 1)/(r - 1) = total: the
+// bisection of the host set-up (same bracket, same stopping rule), with
+// bounded loops for any input
+__device__ double geometric_ratio_dev(int n, double dz1, double total)
+{
+  const double fn = (double)n;
+  if (fabs(fn * dz1 - total) <= 1e-15 * total) return 1.0;
+  const bool grow = fn * dz1 < total;
+  double lo = grow ? 1.0 : 1e-6, hi = grow ? 2.0 : 1.0;
+  auto sum = [&](double r) { return r == 1.0 ? dz1 * fn : dz1 * (pow(r, fn) - 1.0) / (r - 1.0); };
+  for (int i = 0; i < 1100 && sum(hi) < total; ++i) hi = hi * 2.0;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (sum(mid) < total)
+      lo = mid;
+    else
+      hi = mid;
+    if (hi - lo <= 1e-16 * hi) break;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// One column per group of LPC lanes, lane l = layer l.  The group builds
+// the node centres in LDS (lanes stride over the nodes), then each layer
+// interpolates every variable's profile at its own centre.
+template <int LPC>
+__global__ __launch_bounds__(BLOCK) void k_warm_profiles(Dev d, const int *list, int nlist, WarmProfiles w)
+{
+  constexpr int G = BLOCK / LPC;
+  __shared__ double sdz[G][kWarmMaxNodes];
+  __shared__ double szc[G][kWarmMaxNodes];
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int slot = blockIdx.x * G + g;
+  const bool live = slot < nlist;
+  const int c = live ? list[slot] : 0;
+  const int o = live ? d.off[c] : 0, K = live ? d.off[c + 1] - o : 0;
+  if (live) {
+    double depth = 0.0;
+    if (w.depth)
+      depth = w.depth[c];
+    else
+      for (int k = 0; k < K; ++k) depth = depth + d.dz[o + k];
+    const double r = geometric_ratio_dev(w.nz, w.dz1, depth);
+    for (int j = l; j < w.nz; j += LPC) sdz[g][j] = w.dz1 * pow(r, (double)j);
+  }
+  __syncthreads();
+  if (live)
+    for (int j = l; j < w.nz; j += LPC) {
+      double zi = 0.0;
+      for (int q = 0; q < j; ++q) zi = zi + sdz[g][q];
+      szc[g][j] = zi + 0.5 * sdz[g][j];
+    }
+  __syncthreads();
+  if (!live || l >= K) return;
+  double zi = 0.0;
+  for (int q = 0; q < l; ++q) zi = zi + d.dz[o + q];
+  const double zk = zi + 0.5 * d.dz[o + l];
+  int j = -1;
+  for (int q = 0; q < w.nz; ++q) j += zk >= szc[g][q] ? 1 : 0;
+  j = j < 0 ? 0 : (j > w.nz - 2 ? w.nz - 2 : j);
+  const double z0 = szc[g][j], z1 = szc[g][j + 1];
+  double wt = (zk - z0) / (z1 - z0);
+  wt = wt < 0.0 ? 0.0 : (wt > 1.0 ? 1.0 : wt);
+  const float *row = w.prof + (long long)c * w.ld;
+  const long long cell = (long long)o + l;
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    const int i0 = n * w.nz + j;
+    const double f0 = (double)row[i0] * w.scale[i0] + w.offset[i0];
+    const double f1 = (double)row[i0 + 1] * w.scale[i0 + 1] + w.offset[i0 + 1];
+    const double x = pow(10.0, f0 + wt * (f1 - f0)) - w.eps[n];
+    d.conc[n * d.ncell + cell] = x < 0.0 ? 0.0 : x;  // NaN stays NaN
+  }
+}
+
+void launch_warm_profiles(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const WarmProfiles &w)
+{
+  SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_warm_profiles<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n,
+                                          w));
+}
+
 void launch_fill_int(hipStream_t st, int *p, int n, int v)
 {
   if (n <= 0) return;

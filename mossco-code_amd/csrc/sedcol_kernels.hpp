@@ -75,6 +75,22 @@ void launch_emp_begin(hipStream_t st, const int *list, int n, int r0, int *wrec,
 // point to the device arrays e1, e2 (ncell) and cum (ncol) to fill)
 void launch_zw_geometry(hipStream_t st, const Dev &d, const int *list, int n, double k_l, double *e1, double *e2,
                         double *cum);
+// warm start of the listed columns (sedcol_set_state_profiles): prof row c
+// (stride ld floats) holds the normalised log10 profile u of variable n at
+// node j in prof[c ld + n nz + j]; y = u scale[n nz + j] + offset[n nz + j]
+// on the centres of an nz-layer geometric grid from dz1 to the column's
+// depth (depth[c], or the column's sum of dz when null); every layer gets
+// max(10^y(z) - eps[n], 0) at its centre, y linear in depth between nodes
+// and constant beyond the end nodes.  scale, offset, eps, depth: device.
+constexpr int kWarmMaxNodes = 64;
+struct WarmProfiles {
+  const float *prof;
+  long long ld;
+  int nz;
+  double dz1;
+  const double *scale, *offset, *eps, *depth;
+};
+void launch_warm_profiles(int lpc, hipStream_t st, const Dev &d, const int *list, int n, const WarmProfiles &w);
 // v into p[0..n-1] (a kernel, so that it can be captured in a graph)
 void launch_fill_int(hipStream_t st, int *p, int n, int v);
 void launch_fill_dbl(hipStream_t st, double *p, long long n, double v);

@@ -799,11 +799,10 @@ int sedsurr_loss_grad(sedsurr_handle *h, const float *X, const float *Y, int32_t
   return SEDCOL_OK;
 }
 
-int sedsurr_predict(sedsurr_handle *h, const float *X, int64_t n, float *Y)
+// prediction passes of max_batch rows; Y on the device (written by the last
+// layer's product directly) or in host memory (copied per pass)
+static int predict_rows(sedsurr_handle *h, const float *X, int64_t n, float *Y, bool y_on_device)
 {
-  if (!h || !X || !Y) return fail(SEDCOL_EINVAL, "null argument");
-  const DeviceScope dscope(h->device);
-  if (n < 0) return fail(SEDCOL_EINVAL, "need n >= 0");
   int launches = 0;
   HIPOK(hipEventRecord(h->e0, h->st));
   const int B = h->s.max_batch, no = h->s.n_out;
@@ -814,9 +813,11 @@ int sedsurr_predict(sedsurr_handle *h, const float *X, int64_t n, float *Y)
     hipLaunchKernelGGL(ss::k_gather, dim3(b), dim3(64), 0, h->st, h->xin, h->s.n_in, (const float *)nullptr, no,
                        (const int *)nullptr, b, h->kin, h->nout_p, h->xb, (float *)nullptr);
     ++launches;
-    forward(h, b, &launches, h->yb);  // compact b x n_out output in the target buffer
+    // compact b x n_out output, in place on the device or staged for the host
+    forward(h, b, &launches, y_on_device ? Y + r0 * no : h->yb);
     HIPOK(hipGetLastError());
-    HIPOK(hipMemcpyAsync(Y + r0 * no, h->yb, sizeof(float) * (size_t)b * no, hipMemcpyDeviceToHost, h->st));
+    if (!y_on_device)
+      HIPOK(hipMemcpyAsync(Y + r0 * no, h->yb, sizeof(float) * (size_t)b * no, hipMemcpyDeviceToHost, h->st));
   }
   HIPOK(hipEventRecord(h->e1, h->st));
   HIPOK(hipEventSynchronize(h->e1));
@@ -825,6 +826,28 @@ int sedsurr_predict(sedsurr_handle *h, const float *X, int64_t n, float *Y)
   h->last_ms = ms;
   h->last_launches = launches;
   return SEDCOL_OK;
+}
+
+int sedsurr_predict(sedsurr_handle *h, const float *X, int64_t n, float *Y)
+{
+  if (!h || !X || !Y) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
+  if (n < 0) return fail(SEDCOL_EINVAL, "need n >= 0");
+  return predict_rows(h, X, n, Y, false);
+}
+
+int sedsurr_predict_device(sedsurr_handle *h, const float *X, int64_t n, float *Y_device)
+{
+  if (!h || !X || !Y_device) return fail(SEDCOL_EINVAL, "null argument");
+  const DeviceScope dscope(h->device);
+  if (n < 0) return fail(SEDCOL_EINVAL, "need n >= 0");
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, Y_device) != hipSuccess || at.type != hipMemoryTypeDevice ||
+      at.device != h->device) {
+    (void)hipGetLastError();
+    return fail(SEDCOL_EINVAL, "Y_device must be device memory of the handle's device");
+  }
+  return predict_rows(h, X, n, Y_device, true);
 }
 
 int sedsurr_last_timing(sedsurr_handle *h, double *ms, int32_t *launches)

@@ -33,7 +33,8 @@ NBUD = 16
 #: every symbol include/sedcol.h declares
 EXPORTS = (
     "sedcol_create", "sedcol_destroy", "sedcol_last_error", "sedcol_set_params", "sedcol_set_geometry",
-    "sedcol_set_forcing", "sedcol_set_forcing_series", "sedcol_set_state", "sedcol_get_state", "sedcol_rhs",
+    "sedcol_set_forcing", "sedcol_set_forcing_series", "sedcol_set_state", "sedcol_get_state",
+    "sedcol_set_state_profiles", "sedcol_rhs",
     "sedcol_rhs_noalloc", "sedcol_rates",
     "sedcol_step", "sedcol_run", "sedcol_integrate", "sedcol_get_fluxes", "sedcol_get_budget",
     "sedcol_reset_budget", "sedcol_get_inventory", "sedcol_get_diagnostics", "sedcol_reset_diagnostics",
@@ -124,6 +125,7 @@ def lib() -> ctypes.CDLL:
             "sedcol_set_forcing_series": (ctypes.c_int, [vp, i32, dbl, dbl, pd, pd, pd, i32]),
             "sedcol_set_state": (ctypes.c_int, [vp, pd]),
             "sedcol_get_state": (ctypes.c_int, [vp, pd]),
+            "sedcol_set_state_profiles": (ctypes.c_int, [vp, vp, ctypes.c_int64, i32, pd, pd, pd, dbl, pd]),
             "sedcol_rhs": (ctypes.c_int, [vp, pd, pd, pd]),
             "sedcol_rhs_noalloc": (ctypes.c_int, [vp, pd, pd, pd]),
             "sedcol_rates": (ctypes.c_int, [vp, pd, pd]),
@@ -277,6 +279,25 @@ class SedimentColumns:
         x = np.empty(self._state_shape())
         check(lib().sedcol_get_state(self._h, _pd(x)))
         return x
+
+    def set_state_profiles(self, prof, nz, scale, offset, eps, dz1=1.0e-3, depth=None):
+        """Warm start on the device from log10 profiles (sedcol_set_state_profiles).
+        prof: a float32 CUDA tensor (ncol, >= NV*nz), row-contiguous, e.g.
+        from surrogate.Network.predict_device; scale/offset (NV*nz), eps
+        (NV) and depth (ncol, optional) are host arrays."""
+        if (str(prof.dtype) != "torch.float32" or not prof.is_cuda or prof.dim() != 2 or prof.stride(1) != 1
+                or prof.shape[0] != self.ncol or prof.shape[1] < NVAR * nz):
+            raise ValueError("prof must be a float32 CUDA tensor of ncol rows with unit column stride")
+        sc, of = _c(scale), _c(offset)
+        ep = _c(eps)
+        if sc.size != NVAR * nz or of.size != NVAR * nz or ep.size != NVAR:
+            raise ValueError("scale and offset need nvar*nz entries, eps nvar")
+        dp = None if depth is None else _c(depth)
+        if dp is not None and dp.size != self.ncol:
+            raise ValueError("depth needs ncol entries")
+        check(lib().sedcol_set_state_profiles(self._h, ctypes.c_void_p(prof.data_ptr()), int(prof.stride(0)),
+                                              int(nz), _pd(sc), _pd(of), _pd(ep), float(dz1),
+                                              _pd(dp) if dp is not None else None))
 
     def _state_shape(self):
         if self.layout == LAYOUT_PACKED:

@@ -189,11 +189,13 @@ def from_targets(y: np.ndarray, depth, nlayers) -> np.ndarray:
 
 
 def spin(dom: cases.Domain, years: float, tol: float, chunk_days: float = 10.0, conc=None, flux_ref=None,
-         flux_rtol: float = 0.01, floor=None, columns=None, segment_years: float = 0.25, log=None):
+         flux_rtol: float = 0.01, floor=None, columns=None, segment_years: float = 0.25, log=None, init=None):
     """Spin the domain's columns up on the GPU from `conc` (default: the
-    cold state), in segments of segment_years (a progress line each through
-    `log`).  Returns dict(state, t_conv, resid, n_unconverged, flux,
-    wall_s)."""
+    cold state), or from the state that init(handle) sets on the device (a
+    warm start, SedimentColumns.set_state_profiles), in segments of
+    segment_years (a progress line each through `log`).  Returns
+    dict(state, t_conv, resid, n_unconverged, flux, wall_s, init_s), init_s
+    the time of init."""
     import time
 
     from . import capi
@@ -202,6 +204,10 @@ def spin(dom: cases.Domain, years: float, tol: float, chunk_days: float = 10.0, 
     try:
         if conc is not None:
             g.set_state(conc)
+        t_init = time.perf_counter()
+        if init is not None:
+            init(g)
+        t_init = time.perf_counter() - t_init
         d0 = g.diagnostics()
         t0 = time.perf_counter()
         t, t_end = 0.0, years * YEAR
@@ -227,7 +233,7 @@ def spin(dom: cases.Domain, years: float, tol: float, chunk_days: float = 10.0, 
             g.close()
     att = (d1["accepted"] - d0["accepted"]) + (d1["rejected"] - d0["rejected"])
     return {"state": x, "t_conv": tconv, "resid": rs, "n_unconverged": nu, "flux": fl, "wall_s": wall,
-            "attempts": att}
+            "attempts": att, "init_s": t_init}
 
 
 def generate(n: int, seed: int, years: float, tol: float = 1.0e-8, nlayers=K_SURR, tails: float = 0.0,

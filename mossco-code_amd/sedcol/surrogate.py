@@ -24,6 +24,7 @@ SHAPE = (12, 512, 3, 330)  # n_in, n_hidden, hidden layers, n_out
 #: every symbol include/sedsurr.h declares
 EXPORTS = ("sedsurr_create", "sedsurr_destroy", "sedsurr_last_error", "sedsurr_num_params", "sedsurr_set_params",
            "sedsurr_get_params", "sedsurr_set_data", "sedsurr_train", "sedsurr_loss_grad", "sedsurr_predict",
+           "sedsurr_predict_device",
            "sedsurr_last_timing")
 
 
@@ -70,6 +71,7 @@ def lib() -> ctypes.CDLL:
             "sedsurr_train": (ctypes.c_int, [vp, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(Adam), pf]),
             "sedsurr_loss_grad": (ctypes.c_int, [vp, pf, pf, i32, pf, pf]),
             "sedsurr_predict": (ctypes.c_int, [vp, pf, i64, pf]),
+            "sedsurr_predict_device": (ctypes.c_int, [vp, pf, i64, vp]),
             "sedsurr_last_timing": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]),
         }
         for name, (res, args) in sig.items():
@@ -161,6 +163,21 @@ class Network:
         Y = np.empty((X.shape[0], self.shape[3]), dtype=np.float32)
         check(lib().sedsurr_predict(self._h, _pf(X), X.shape[0], _pf(Y)))
         return Y
+
+    def predict_device(self, X, out=None):
+        """predict with the output left on the GPU: a float32 CUDA tensor
+        (n, n_out) (allocated when out is None), ready for
+        SedimentColumns.set_state_profiles."""
+        import torch
+
+        X = _f32(X)
+        n, no = X.shape[0], self.shape[3]
+        if out is None:
+            out = torch.empty((n, no), dtype=torch.float32, device="cuda")
+        if out.dtype != torch.float32 or not out.is_cuda or tuple(out.shape) != (n, no) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float32 CUDA tensor of shape (n, n_out)")
+        check(lib().sedsurr_predict_device(self._h, _pf(X), n, ctypes.c_void_p(out.data_ptr())))
+        return out
 
     def last_timing(self):
         ms = ctypes.c_double()
@@ -267,6 +284,18 @@ def fit(X, Y, *, epochs=400, batch=4096, lr=1e-3, lr_final=1e-5, val_fraction=0.
     return model, hist
 
 
+def warm_init(model: Model, yn_dev, depth, dz1: float = 1.0e-3):
+    """init for spinup.spin: the network's normalised prediction yn_dev (a
+    CUDA tensor, Network.predict_device) denormalised, interpolated to every
+    column's layers and set as its state, on the device
+    (sedcol_set_state_profiles).  depth: the columns' total depths (m), the
+    depth of the network's 30-layer grid."""
+    from .spinup import EPS, K_SURR
+
+    depth = np.asarray(depth, dtype=np.float64)
+    return lambda g: g.set_state_profiles(yn_dev, K_SURR, model.y_std, model.y_mean, EPS, dz1, depth)
+
+
 def predict(model: Model, X, net: Network | None = None):
     """Log10 profiles (n x 330) for raw inputs X (n x 12)."""
     own = net is None
@@ -325,17 +354,15 @@ def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_
     net = Network(model.shape, max_batch=4096)
     try:
         net.set_params(model.params)
-        yn = net.predict((X - model.x_mean) / model.x_std)
+        yn = net.predict_device((X - model.x_mean) / model.x_std)
         pred_ms, _ = net.last_timing()
     finally:
         net.close()
-    y = yn.astype(np.float64) * model.y_std + model.y_mean
     t_pred = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    x0 = spinup.from_targets(y, s["depth"], K)
-    t_interp = time.perf_counter() - t0
-    warm = spinup.spin(dom, warm_days / 365.0, res_tol, chunk_days=chunk_days, conc=x0, flux_ref=F_cold,
+    init = warm_init(model, yn, s["depth"])
+    warm = spinup.spin(dom, warm_days / 365.0, res_tol, chunk_days=chunk_days, init=init, flux_ref=F_cold,
                        flux_rtol=flux_rtol, floor=floor, log=say)
+    t_interp = warm["init_s"]
     fb = warm["t_conv"] < 0.0
     t_fb = 0.0
     if fb.any():
@@ -345,8 +372,12 @@ def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_
     wall_warm = t_pred + t_interp + warm["wall_s"] + t_fb
 
     say("residual-only retirement (accuracy figure)")
-    res_only = spinup.spin(dom, warm_days / 365.0, res_tol, chunk_days=chunk_days, conc=x0, log=say)
+    res_only = spinup.spin(dom, warm_days / 365.0, res_tol, chunk_days=chunk_days, init=init, log=say)
     ro = res_only["t_conv"] >= 0.0
+    with capi.SedimentColumns.from_domain(dom) as g:
+        init(g)
+        x0 = g.get_state()
+    del yn
 
     lg = lambda a: np.log10(np.maximum(a, 0.0) + spinup.EPS[:, None])
     prof_err = np.abs(lg(x0) - lg(cold["state"]))

@@ -1,8 +1,10 @@
 """The surrogate configurations at their stated sizes, on the GPU.
 
   * warm start of 1e6 ragged columns (layers U{20..40}, 5 % of the inputs
-    outside the training ranges): prediction, interpolation to each
-    column's layers, one simulated hour of integration with retirement by
+    outside the training ranges): prediction and interpolation to each
+    column's layers on the device (sedsurr_predict_device,
+    sedcol_set_state_profiles; 2000 sampled columns against the host path
+    within 1e-12 relative), one simulated hour of integration with retirement by
     the steady-state residual; every concentration finite and non-negative,
     the C/N/P budget of every column closed to 1e-12, and the retirement of
     each column consistent with its criterion (retired exactly when its
@@ -34,15 +36,32 @@ def test_warm_start_one_million_columns(gpu_ready):
     dom = spinup.build(s, K)
     print(f"built {n} columns, {dom.ncell} cells ({time.perf_counter() - t0:.1f} s)", flush=True)
     model = surrogate.Model.load(MODEL)
-    y = surrogate.predict(model, spinup.as_matrix(s))
-    assert y.shape == (n, 330) and np.all(np.isfinite(y))
-    x0 = spinup.from_targets(y, s["depth"], K)
-    del y
-    print(f"predicted and interpolated ({time.perf_counter() - t0:.1f} s)", flush=True)
-    assert np.all(np.isfinite(x0)) and np.all(x0 >= 0.0)
+    Xn = (spinup.as_matrix(s) - model.x_mean) / model.x_std
+    with surrogate.Network(model.shape, max_batch=4096) as net:
+        net.set_params(model.params)
+        yn = net.predict_device(Xn)
+        # host path on a sample of rows, for the comparison below
+        sample = np.sort(rng.choice(n, 2000, replace=False))
+        yn_host = net.predict(Xn[sample])
+    assert tuple(yn.shape) == (n, 330)
+    assert np.array_equal(yn[sample].cpu().numpy(), yn_host)
     tol = 1.0e-6
     with capi.SedimentColumns.from_domain(dom) as g:
-        g.set_state(x0)
+        t1 = time.perf_counter()
+        surrogate.warm_init(model, yn, s["depth"])(g)
+        print(f"predicted on the device; state interpolated on the device in {time.perf_counter() - t1:.3f} s "
+              f"({time.perf_counter() - t0:.1f} s)", flush=True)
+        del yn
+        x0 = g.get_state()
+        assert np.all(np.isfinite(x0)) and np.all(x0 >= 0.0)
+        # the sampled columns against the host interpolation (the node grids
+        # may differ in the last bit: pow of the device and of the host)
+        off = dom.off
+        cells = np.concatenate([np.arange(off[c], off[c + 1]) for c in sample])
+        ref = spinup.from_targets(yn_host.astype(np.float64) * model.y_std + model.y_mean, s["depth"][sample],
+                                  K[sample])
+        assert np.all(np.abs(x0[:, cells] - ref) <= 1e-12 * np.abs(ref) + 1e-14)
+        del x0
         inv0 = g.inventory()
         tc, rs, nu = g.spinup(0.0, 3600.0, 3600.0, tol, spinup.CFLOOR)
         x = g.get_state()
