@@ -73,10 +73,13 @@ int sedsurr_loss_grad(sedsurr_handle *h, const float *X, const float *Y, int32_t
 
 /* Y (n x n_out) = network(X (n x n_in)), normalised units. */
 int sedsurr_predict(sedsurr_handle *h, const float *X, int64_t n, float *Y);
-/* The same with Y in device memory of the handle's device (for example the
- * input of sedcol_set_state_profiles, so that a warm start never passes
- * through host memory).  X stays host memory. */
-int sedsurr_predict_device(sedsurr_handle *h, const float *X, int64_t n, float *Y_device);
+/* The same with Y left on the GPU, for example as the input of
+ * sedcol_set_state_profiles, so that a warm start never passes through host
+ * memory.  X is host memory; *Y_device receives the address of a device
+ * buffer of the handle (n x n_out, row-major) that stays valid until the
+ * next sedsurr_predict_device call or sedsurr_destroy.  The buffer grows to
+ * the largest n asked for and is freed with the handle. */
+int sedsurr_predict_device(sedsurr_handle *h, const float *X, int64_t n, const float **Y_device);
 
 /* Timing of the last sedsurr_train / sedsurr_predict (HIP events on the
  * handle's stream): milliseconds and kernel launches. */

@@ -413,6 +413,8 @@ struct sedsurr_handle {
   int *rows = nullptr;
   long long rows_cap = 0;
   float *xin = nullptr, *yin = nullptr; // staging of loss_grad / predict inputs
+  float *ydev = nullptr;                // sedsurr_predict_device output (ydev_rows x n_out)
+  int64_t ydev_rows = 0;
   long long step = 0;                   // Adam steps taken
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -652,7 +654,8 @@ int sedsurr_destroy(sedsurr_handle *h)
   const DeviceScope dscope(h->device);
   if (h->st) (void)hipStreamSynchronize(h->st);
   std::vector<void *> ptrs = {h->w, h->g, h->m, h->v, h->wb, h->xb, h->yh, h->yb, h->dz0, h->dz1,
-                              h->part, h->loss, h->X, h->Y, h->rows, h->xin, h->yin, h->kpart, h->kpart_all};
+                              h->part, h->loss, h->X, h->Y, h->rows, h->xin, h->yin, h->kpart, h->kpart_all,
+                              h->ydev};
   for (auto *q : h->hs) ptrs.push_back(q);
   for (auto *q : h->zs) ptrs.push_back(q);
   for (void *q : ptrs)
@@ -836,18 +839,22 @@ int sedsurr_predict(sedsurr_handle *h, const float *X, int64_t n, float *Y)
   return predict_rows(h, X, n, Y, false);
 }
 
-int sedsurr_predict_device(sedsurr_handle *h, const float *X, int64_t n, float *Y_device)
+int sedsurr_predict_device(sedsurr_handle *h, const float *X, int64_t n, const float **Y_device)
 {
   if (!h || !X || !Y_device) return fail(SEDCOL_EINVAL, "null argument");
   const DeviceScope dscope(h->device);
-  if (n < 0) return fail(SEDCOL_EINVAL, "need n >= 0");
-  hipPointerAttribute_t at{};
-  if (hipPointerGetAttributes(&at, Y_device) != hipSuccess || at.type != hipMemoryTypeDevice ||
-      at.device != h->device) {
-    (void)hipGetLastError();
-    return fail(SEDCOL_EINVAL, "Y_device must be device memory of the handle's device");
+  if (n < 1) return fail(SEDCOL_EINVAL, "need n >= 1");
+  if (n > h->ydev_rows) {
+    HIPOK(hipStreamSynchronize(h->st));
+    if (h->ydev) HIPOK(hipFree(h->ydev));
+    h->ydev = nullptr;
+    h->ydev_rows = 0;
+    HIPOK(hipMalloc(&h->ydev, sizeof(float) * (size_t)n * h->s.n_out));
+    h->ydev_rows = n;
   }
-  return predict_rows(h, X, n, Y_device, true);
+  const int rc = predict_rows(h, X, n, h->ydev, true);
+  *Y_device = rc == SEDCOL_OK ? h->ydev : nullptr;
+  return rc;
 }
 
 int sedsurr_last_timing(sedsurr_handle *h, double *ms, int32_t *launches)

@@ -17,6 +17,7 @@ from __future__ import annotations
 import ctypes
 import os
 from pathlib import Path
+from typing import NamedTuple
 
 import numpy as np
 
@@ -44,6 +45,13 @@ EXPORTS = (
     "sedcol_assemble_forcing", "sedcol_set_flux_cap", "sedcol_set_bioturbation_zw",
 )
 NBIN = 3
+
+
+class DeviceRows(NamedTuple):
+    """rows x ld float32 values in device memory (row-major), owned elsewhere."""
+    ptr: int
+    rows: int
+    ld: int
 
 
 class SedcolError(RuntimeError):
@@ -280,14 +288,13 @@ class SedimentColumns:
         check(lib().sedcol_get_state(self._h, _pd(x)))
         return x
 
-    def set_state_profiles(self, prof, nz, scale, offset, eps, dz1=1.0e-3, depth=None):
+    def set_state_profiles(self, prof: DeviceRows, nz, scale, offset, eps, dz1=1.0e-3, depth=None):
         """Warm start on the device from log10 profiles (sedcol_set_state_profiles).
-        prof: a float32 CUDA tensor (ncol, >= NV*nz), row-contiguous, e.g.
-        from surrogate.Network.predict_device; scale/offset (NV*nz), eps
+        prof: DeviceRows of ncol float32 rows in device memory (for example
+        from surrogate.Network.predict_device); scale/offset (NV*nz), eps
         (NV) and depth (ncol, optional) are host arrays."""
-        if (str(prof.dtype) != "torch.float32" or not prof.is_cuda or prof.dim() != 2 or prof.stride(1) != 1
-                or prof.shape[0] != self.ncol or prof.shape[1] < NVAR * nz):
-            raise ValueError("prof must be a float32 CUDA tensor of ncol rows with unit column stride")
+        if prof.rows != self.ncol or prof.ld < NVAR * nz:
+            raise ValueError("prof must hold ncol rows of at least nvar*nz values")
         sc, of = _c(scale), _c(offset)
         ep = _c(eps)
         if sc.size != NVAR * nz or of.size != NVAR * nz or ep.size != NVAR:
@@ -295,7 +302,7 @@ class SedimentColumns:
         dp = None if depth is None else _c(depth)
         if dp is not None and dp.size != self.ncol:
             raise ValueError("depth needs ncol entries")
-        check(lib().sedcol_set_state_profiles(self._h, ctypes.c_void_p(prof.data_ptr()), int(prof.stride(0)),
+        check(lib().sedcol_set_state_profiles(self._h, ctypes.c_void_p(prof.ptr), int(prof.ld),
                                               int(nz), _pd(sc), _pd(of), _pd(ep), float(dz1),
                                               _pd(dp) if dp is not None else None))
 

@@ -71,7 +71,7 @@ def lib() -> ctypes.CDLL:
             "sedsurr_train": (ctypes.c_int, [vp, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(Adam), pf]),
             "sedsurr_loss_grad": (ctypes.c_int, [vp, pf, pf, i32, pf, pf]),
             "sedsurr_predict": (ctypes.c_int, [vp, pf, i64, pf]),
-            "sedsurr_predict_device": (ctypes.c_int, [vp, pf, i64, vp]),
+            "sedsurr_predict_device": (ctypes.c_int, [vp, pf, i64, ctypes.POINTER(vp)]),
             "sedsurr_last_timing": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]),
         }
         for name, (res, args) in sig.items():
@@ -164,20 +164,17 @@ class Network:
         check(lib().sedsurr_predict(self._h, _pf(X), X.shape[0], _pf(Y)))
         return Y
 
-    def predict_device(self, X, out=None):
-        """predict with the output left on the GPU: a float32 CUDA tensor
-        (n, n_out) (allocated when out is None), ready for
+    def predict_device(self, X):
+        """predict with the output left on the GPU, in a buffer of this
+        network that stays valid until its next predict_device or close:
+        capi.DeviceRows (n rows of n_out), ready for
         SedimentColumns.set_state_profiles."""
-        import torch
+        from .capi import DeviceRows
 
         X = _f32(X)
-        n, no = X.shape[0], self.shape[3]
-        if out is None:
-            out = torch.empty((n, no), dtype=torch.float32, device="cuda")
-        if out.dtype != torch.float32 or not out.is_cuda or tuple(out.shape) != (n, no) or not out.is_contiguous():
-            raise ValueError("out must be a contiguous float32 CUDA tensor of shape (n, n_out)")
-        check(lib().sedsurr_predict_device(self._h, _pf(X), n, ctypes.c_void_p(out.data_ptr())))
-        return out
+        ptr = ctypes.c_void_p()
+        check(lib().sedsurr_predict_device(self._h, _pf(X), X.shape[0], ctypes.byref(ptr)))
+        return DeviceRows(int(ptr.value), int(X.shape[0]), int(self.shape[3]))
 
     def last_timing(self):
         ms = ctypes.c_double()
@@ -285,11 +282,11 @@ def fit(X, Y, *, epochs=400, batch=4096, lr=1e-3, lr_final=1e-5, val_fraction=0.
 
 
 def warm_init(model: Model, yn_dev, depth, dz1: float = 1.0e-3):
-    """init for spinup.spin: the network's normalised prediction yn_dev (a
-    CUDA tensor, Network.predict_device) denormalised, interpolated to every
-    column's layers and set as its state, on the device
-    (sedcol_set_state_profiles).  depth: the columns' total depths (m), the
-    depth of the network's 30-layer grid."""
+    """init for spinup.spin: the network's normalised prediction yn_dev
+    (capi.DeviceRows from Network.predict_device, whose network must stay
+    open) denormalised, interpolated to every column's layers and set as its
+    state, on the device (sedcol_set_state_profiles).  depth: the columns'
+    total depths (m), the depth of the network's 30-layer grid."""
     from .spinup import EPS, K_SURR
 
     depth = np.asarray(depth, dtype=np.float64)
@@ -351,33 +348,33 @@ def evaluate(model: Model, n: int, seed: int, *, years=2.0, warm_days=30.0, res_
     say("warm: prediction and integration")
     X = spinup.as_matrix(s)
     t0 = time.perf_counter()
+    # the network holds the prediction on the device until it is closed
     net = Network(model.shape, max_batch=4096)
     try:
         net.set_params(model.params)
         yn = net.predict_device((X - model.x_mean) / model.x_std)
         pred_ms, _ = net.last_timing()
+        t_pred = time.perf_counter() - t0
+        init = warm_init(model, yn, s["depth"])
+        warm = spinup.spin(dom, warm_days / 365.0, res_tol, chunk_days=chunk_days, init=init, flux_ref=F_cold,
+                           flux_rtol=flux_rtol, floor=floor, log=say)
+        t_interp = warm["init_s"]
+        fb = warm["t_conv"] < 0.0
+        t_fb = 0.0
+        if fb.any():
+            say(f"fallback: cold spin-up of {int(fb.sum())} columns")
+            r = spinup.spin(dom.subset(np.flatnonzero(fb)), years, 1e-300, chunk_days=10.0, log=say)
+            t_fb = r["wall_s"]
+        wall_warm = t_pred + t_interp + warm["wall_s"] + t_fb
+
+        say("residual-only retirement (accuracy figure)")
+        res_only = spinup.spin(dom, warm_days / 365.0, res_tol, chunk_days=chunk_days, init=init, log=say)
+        ro = res_only["t_conv"] >= 0.0
+        with capi.SedimentColumns.from_domain(dom) as g:
+            init(g)
+            x0 = g.get_state()
     finally:
         net.close()
-    t_pred = time.perf_counter() - t0
-    init = warm_init(model, yn, s["depth"])
-    warm = spinup.spin(dom, warm_days / 365.0, res_tol, chunk_days=chunk_days, init=init, flux_ref=F_cold,
-                       flux_rtol=flux_rtol, floor=floor, log=say)
-    t_interp = warm["init_s"]
-    fb = warm["t_conv"] < 0.0
-    t_fb = 0.0
-    if fb.any():
-        say(f"fallback: cold spin-up of {int(fb.sum())} columns")
-        r = spinup.spin(dom.subset(np.flatnonzero(fb)), years, 1e-300, chunk_days=10.0, log=say)
-        t_fb = r["wall_s"]
-    wall_warm = t_pred + t_interp + warm["wall_s"] + t_fb
-
-    say("residual-only retirement (accuracy figure)")
-    res_only = spinup.spin(dom, warm_days / 365.0, res_tol, chunk_days=chunk_days, init=init, log=say)
-    ro = res_only["t_conv"] >= 0.0
-    with capi.SedimentColumns.from_domain(dom) as g:
-        init(g)
-        x0 = g.get_state()
-    del yn
 
     lg = lambda a: np.log10(np.maximum(a, 0.0) + spinup.EPS[:, None])
     prof_err = np.abs(lg(x0) - lg(cold["state"]))

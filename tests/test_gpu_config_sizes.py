@@ -37,21 +37,18 @@ def test_warm_start_one_million_columns(gpu_ready):
     print(f"built {n} columns, {dom.ncell} cells ({time.perf_counter() - t0:.1f} s)", flush=True)
     model = surrogate.Model.load(MODEL)
     Xn = (spinup.as_matrix(s) - model.x_mean) / model.x_std
-    with surrogate.Network(model.shape, max_batch=4096) as net:
-        net.set_params(model.params)
-        yn = net.predict_device(Xn)
-        # host path on a sample of rows, for the comparison below
-        sample = np.sort(rng.choice(n, 2000, replace=False))
-        yn_host = net.predict(Xn[sample])
-    assert tuple(yn.shape) == (n, 330)
-    assert np.array_equal(yn[sample].cpu().numpy(), yn_host)
+    sample = np.sort(rng.choice(n, 2000, replace=False))
     tol = 1.0e-6
-    with capi.SedimentColumns.from_domain(dom) as g:
+    with surrogate.Network(model.shape, max_batch=4096) as net, capi.SedimentColumns.from_domain(dom) as g:
+        net.set_params(model.params)
+        # host path on a sample of rows, for the comparison below
+        yn_host = net.predict(Xn[sample])
+        yn = net.predict_device(Xn)
+        assert (yn.rows, yn.ld) == (n, 330)
         t1 = time.perf_counter()
         surrogate.warm_init(model, yn, s["depth"])(g)
         print(f"predicted on the device; state interpolated on the device in {time.perf_counter() - t1:.3f} s "
               f"({time.perf_counter() - t0:.1f} s)", flush=True)
-        del yn
         x0 = g.get_state()
         assert np.all(np.isfinite(x0)) and np.all(x0 >= 0.0)
         # the sampled columns against the host interpolation (the node grids
