@@ -95,6 +95,23 @@ def test_rk4_open_top_distributed_flux(gpu_ready, tmp_path, profile):
     assert np.array_equal(d["fluxes"][5:10], orc.flux[5:10, 0])
 
 
+@pytest.mark.parametrize("method", [0, 3, 4, 5])
+def test_other_fixed_step_methods(gpu_ready, tmp_path, method):
+    """EULER, RK4-3/8 and the EMP-1/EMP-2 steppers through the same
+    namelist key, bitwise like the oracle."""
+    s, d, lines = _run_both(tmp_path, 70, run={"ode_method": str(method), "bcup_dissolved_variables": "2",
+                                                "output": "25", "dt": "60."})
+    assert s["ode_method"] == method and s["dt"] == 60.0
+    dom = domain_from_setup(s, _params(s))
+    orc = O.Oracle(dom, dom.params.si())
+    x = dom.conc0.copy()
+    for t in (25, 50):
+        assert orc.run(x, s["dt"], 25, method) == 0
+        _check_block(lines, t * s["dt"], s, x)
+    assert orc.run(x, s["dt"], 20, method) == 0
+    assert np.array_equal(d["conc"], x)
+
+
 @pytest.mark.parametrize("method", [6, 7])
 def test_adaptive_emp_and_ros2(gpu_ready, tmp_path, method):
     """ode_method 6 (adaptive EMP) and 7 (ROS2), which the reference's
