@@ -46,7 +46,7 @@ DAY = 86400.0
 DEFAULT_TRAFFIC_BYTES = (458544 * 2.0 + 273073 * 1.0) * 1024
 
 
-def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads, cols_per_thread=32):
+def cpu_baseline(dom, state0, day0, ndays, seconds, nthreads, cols_per_thread=64):
     """Oracle (plain C, OpenMP over columns) on a sample of wet columns,
     starting from the same state; stops after `seconds` of work."""
     sys.path.insert(0, str(REPO / "oracle"))
@@ -301,7 +301,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5, help="untimed simulated days first")
     ap.add_argument("--n", type=int, default=300, help="grid is n x n columns")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--arith", type=int, default=1, help="0 reference operation order, 1 fast form")
     ap.add_argument("--emp-weights", type=int, default=0,
@@ -443,7 +443,7 @@ def main(argv=None):
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(dom, state_w, args.warmup, args.steps, args.cpu_seconds, nthreads)
         # the same oracle on one host core, a smaller sample of the same days
-        one = cpu_baseline(dom, state_w, args.warmup, args.steps, args.cpu_seconds / 2, 1, cols_per_thread=24)
+        one = cpu_baseline(dom, state_w, args.warmup, args.steps, args.cpu_seconds / 2, 1, cols_per_thread=48)
         out["cpu_baseline"]["one_core"] = {k: one[k] for k in ("value", "unit", "cores", "sample")}
     print(json.dumps(out), flush=True)
     return 0
