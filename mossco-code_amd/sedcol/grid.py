@@ -32,8 +32,9 @@ def init_grid_linear(K: int, dzmin: float):
 
 def geometric_ratio(K: int, dz1: float, total: float) -> float:
     """Ratio r with dz1 (r^K - 1)/(r - 1) = total (bisection to full
-    precision)."""
-    if abs(K * dz1 - total) <= 1e-15 * total:
+    precision).  A single layer has no ratio (its layer is dz1 whatever the
+    total): 1."""
+    if K <= 1 or abs(K * dz1 - total) <= 1e-15 * total:
         return 1.0
     def s(r):
         return dz1 * K if r == 1.0 else dz1 * (r ** K - 1.0) / (r - 1.0)
@@ -59,7 +60,7 @@ def geometric_ratios(K, dz1: float, total):
     total = np.asarray(total, dtype=np.float64)
     K, total = np.broadcast_arrays(K, total)
     out = np.ones(K.shape)
-    uniform = np.abs(K * dz1 - total) <= 1e-15 * total
+    uniform = (K <= 1) | (np.abs(K * dz1 - total) <= 1e-15 * total)
     todo = ~uniform
     if not todo.any():
         return out
