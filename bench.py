@@ -310,7 +310,9 @@ def main(argv=None):
                     help="shelf: the benchmark line; ragged: schedule/autotuner comparison on ragged columns; "
                          "rk4: the plain 4x4 RK4 case beside the reference's own stepper on the CPU; "
                          "spinup: surrogate warm start against cold spin-up (wall-time ratio)")
-    ap.add_argument("--spin-columns", type=int, default=4000, help="spinup: columns")
+    ap.add_argument("--spin-columns", type=int, default=100000,
+                    help="spinup: columns (1e5: the run is bound by its total work, about 11 minutes; "
+                         "4000 fits on the GPU at once, so the slowest column sets the wall time)")
     ap.add_argument("--spin-years", type=float, default=2.0, help="spinup: cold presimulation (years)")
     ap.add_argument("--spin-method", type=int, default=7, help="spinup: 6 adaptive EMP, 7 linearly implicit")
     ap.add_argument("--adaptive-method", type=int, default=6,
