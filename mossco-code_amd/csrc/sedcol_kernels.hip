@@ -1792,26 +1792,29 @@ void launch_emp_begin(hipStream_t st, const int *list, int n, int r0, int *wrec,
 // warm start from log10 profiles on a geometric node grid
 // ---------------------------------------------------------------------------
 
-// ratio r of an n-layer geometric grid, dz1 (r^n - 1)/(r - 1) = total: the
-// bisection of the host set-up (same bracket, same stopping rule), with
-// bounded loops for any input
+// ratio r of an n-layer geometric grid, dz1 (r^n - 1)/(r - 1) = total, by
+// Newton's method on g(r) = dz1 (r^n - 1) - total (r - 1), which is convex
+// for r > 0 with the roots 1 and the ratio.  A growing grid (n dz1 < total)
+// has the larger root: from r0 = (total/dz1)^(1/(n-1)), where g(r0) =
+// total - dz1 > 0, the iterates decrease monotonically onto it.  A
+// shrinking grid has the smaller root: from r = 0 they increase onto it.
+// The iteration stops when an iterate no longer moves in its direction
+// (at most a few ulp from the root; the host's bisection lands as close).
 __device__ double geometric_ratio_dev(int n, double dz1, double total)
 {
   const double fn = (double)n;
-  if (fabs(fn * dz1 - total) <= 1e-15 * total) return 1.0;
+  if (n <= 1 || fabs(fn * dz1 - total) <= 1e-15 * total) return 1.0;
   const bool grow = fn * dz1 < total;
-  double lo = grow ? 1.0 : 1e-6, hi = grow ? 2.0 : 1.0;
-  auto sum = [&](double r) { return r == 1.0 ? dz1 * fn : dz1 * (pow(r, fn) - 1.0) / (r - 1.0); };
-  for (int i = 0; i < 1100 && sum(hi) < total; ++i) hi = hi * 2.0;
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (sum(mid) < total)
-      lo = mid;
-    else
-      hi = mid;
-    if (hi - lo <= 1e-16 * hi) break;
+  double r = grow ? pow(total / dz1, 1.0 / (fn - 1.0)) : 0.0;
+  for (int it = 0; it < 100; ++it) {
+    const double rm = pow(r, fn - 1.0);  // r^(n-1)
+    const double g = dz1 * (rm * r - 1.0) - total * (r - 1.0);
+    const double dg = (dz1 * fn) * rm - total;
+    const double rn = r - g / dg;
+    if (!(grow ? rn < r : rn > r)) break;
+    r = rn;
   }
-  return 0.5 * (lo + hi);
+  return r;
 }
 
 // One column per group of LPC lanes, lane l = layer l.  The group builds
@@ -1862,7 +1865,7 @@ __global__ __launch_bounds__(BLOCK) void k_warm_profiles(Dev d, const int *list,
     const int i0 = n * w.nz + j;
     const double f0 = (double)row[i0] * w.scale[i0] + w.offset[i0];
     const double f1 = (double)row[i0 + 1] * w.scale[i0 + 1] + w.offset[i0 + 1];
-    const double x = pow(10.0, f0 + wt * (f1 - f0)) - w.eps[n];
+    const double x = exp10(f0 + wt * (f1 - f0)) - w.eps[n];
     d.conc[n * d.ncell + cell] = x < 0.0 ? 0.0 : x;  // NaN stays NaN
   }
 }

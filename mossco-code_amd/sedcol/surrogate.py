@@ -437,6 +437,11 @@ def _main(argv=None):
     b.add_argument("--steps", type=int, default=200)
     b.add_argument("--batch", type=int, default=4096)
     b.add_argument("--rows", type=int, default=1 << 20, help="rows of one prediction call")
+    w = sub.add_parser("warm", help="time the device warm start (prediction and interpolation) of n ragged columns")
+    w.add_argument("--model", required=True)
+    w.add_argument("--n", type=int, default=1_000_000)
+    w.add_argument("--seed", type=int, default=21)
+    w.add_argument("--repeats", type=int, default=4)
     e = sub.add_parser("evaluate")
     e.add_argument("--model", required=True)
     e.add_argument("--n", type=int, default=10000)
@@ -480,6 +485,30 @@ def _main(argv=None):
                           "launches_per_step": launches / a.steps, "train_tflops": fl / (ms * 1e-3) / 1e12,
                           "predict_rows": a.rows, "predict_ms": pms,
                           "predict_tflops": flops_per_sample() * a.rows / (pms * 1e-3) / 1e12}), flush=True)
+    elif a.cmd == "warm":
+        import time
+
+        from . import capi, spinup
+
+        model = Model.load(a.model)
+        rng = np.random.Generator(np.random.Philox(a.seed))
+        s = spinup.draw_inputs(rng, a.n, tails=0.05)
+        dom = spinup.build(s, rng.integers(20, 41, size=a.n))
+        Xn = (spinup.as_matrix(s) - model.x_mean) / model.x_std
+        with Network(model.shape, max_batch=4096) as net, capi.SedimentColumns.from_domain(dom) as g:
+            net.set_params(model.params)
+            t0 = time.perf_counter()
+            yn = net.predict_device(Xn)
+            t_pred = time.perf_counter() - t0
+            pred_ms, _ = net.last_timing()
+            init = warm_init(model, yn, s["depth"])
+            walls = []
+            for _ in range(a.repeats):
+                t0 = time.perf_counter()
+                init(g)
+                walls.append(time.perf_counter() - t0)
+        print(json.dumps({"columns": a.n, "cells": dom.ncell, "predict_s": t_pred, "predict_kernel_ms": pred_ms,
+                          "set_state_profiles_s": walls}), flush=True)
     else:
         model = Model.load(a.model)
         r = evaluate(model, a.n, a.seed, years=a.years, warm_days=a.warm_days, adaptive_method=a.method,
