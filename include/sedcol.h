@@ -283,7 +283,12 @@ int sedcol_step(sedcol_handle *h, double dt, int32_t method, double *conc_inout,
 /* nsteps fixed steps (methods 0..5) of dt on the device-resident state
  * (sedcol_set_state), steps_per_launch steps per kernel launch (0 = all in
  * one persistent launch).  Equivalent to nsteps ode_solver calls plus the
- * caller's clamp, the Run loop of fabm_sediment_component.F90:1700-1769. */
+ * caller's clamp, the Run loop of fabm_sediment_component.F90:1700-1769.
+ * ADAPTIVE_EULER runs in one workgroup when the wet columns fit one (at
+ * most 32, 16 or 8 columns of up to 16, 32 or 64 layers), in launches of
+ * 4096 steps of at most 2^22 attempts each (SEDCOL_ELIMIT beyond), and
+ * otherwise as separate launches per attempt; steps_per_launch does not
+ * apply to it.  Both give the same results. */
 int sedcol_run(sedcol_handle *h, double dt, int32_t nsteps, int32_t method,
                int32_t steps_per_launch);
 

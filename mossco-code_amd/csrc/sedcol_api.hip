@@ -35,6 +35,9 @@ int fail(int code, const std::string &msg)
   } while (0)
 
 constexpr int kLpc[3] = {16, 32, 64};
+// steps per launch of the one-workgroup adaptive Euler (at most
+// kAeFusedMaxAttempts attempts per launch, i.e. about 1000 per step)
+constexpr int kAeFusedSteps = 4096;
 
 // Workspace of sedcol_rhs_noalloc: staging of its input state, its
 // right-hand side and top fluxes, and (Fortran layout) of the caller's
@@ -1119,7 +1122,35 @@ int sedcol_run(sedcol_handle *h, double dt, int32_t nsteps, int32_t method, int3
   HIPOK(hipMemsetAsync(h->d_flag, 0, sizeof(int), h->stream));
   int launches = 0;
   HIPOK(hipEventRecord(h->ev0, h->stream));
-  if (method == SEDCOL_ADAPTIVE_EULER) {
+  int one_bin = -1, nbins = 0;
+  for (int b = 0; b < 3; ++b)
+    if (!h->bins[b].empty()) {
+      ++nbins;
+      one_bin = b;
+    }
+  if (method == SEDCOL_ADAPTIVE_EULER && nbins == 1 &&
+      (int)h->bins[one_bin].size() <= sc::kAeFusedBlock / kLpc[one_bin]) {
+    // every wet column in one workgroup: the steps run without a host round
+    // trip per attempt, in launches of kAeFusedSteps steps
+    h->h_devview = h->dev();
+    h->h_params = h->p;
+    HIPOK(hipMemcpyAsync(h->d_devview, &h->h_devview, sizeof(Dev), hipMemcpyHostToDevice, h->stream));
+    HIPOK(hipMemcpyAsync(h->d_params, &h->h_params, sizeof(sedcol_params), hipMemcpyHostToDevice, h->stream));
+    for (int s0 = 0; s0 < nsteps; s0 += kAeFusedSteps) {
+      const int ns = std::min(kAeFusedSteps, nsteps - s0);
+      sc::launch_ae_fused(kLpc[one_bin], h->stream, h->d_devview, h->d_list[one_bin], (int)h->bins[one_bin].size(),
+                          h->p, h->d_params, 0, dt, ns, h->d_flag, h->d_small);
+      ++launches;
+      HIPOK(hipGetLastError());
+      double mind = INFINITY;
+      HIPOK(hipMemcpyAsync(&mind, h->d_small, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+      HIPOK(hipStreamSynchronize(h->stream));
+      if (mind < h->last_min_dt) h->last_min_dt = mind;
+      int flag = 0;
+      if ((rc = read_flag(h, &flag))) return rc;
+      if (flag) break;
+    }
+  } else if (method == SEDCOL_ADAPTIVE_EULER) {
     for (int s = 0; s < nsteps; ++s) {
       if ((rc = adaptive_euler(h, dt, 0, &launches))) return rc;
       if (h->p.clamp) {

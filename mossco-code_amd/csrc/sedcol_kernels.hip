@@ -1394,6 +1394,153 @@ __global__ __launch_bounds__(BLOCK) void k_ae_commit(Dev d, const int *list, int
   kadd(d.cellacc[d.ncell + cell], d.cellacc[6 * d.ncell + cell], dt_red * s_in[d.ncell + cell]);
 }
 
+// ADAPTIVE_EULER steps of a small domain in one workgroup: every wet column
+// of the list is a lane group of the block, so the reference's global rule
+// (the step is quartered while any cell of the domain fails the
+// relative-change test, solver_library.F90:100-140) is a block-wide vote
+// instead of a flag read by the host.  Per attempt the same operations as
+// k_rhs_store, k_ae_check and k_ae_commit, and k_clamp after each step, so
+// the results are those of the multi-launch path bit for bit.  The state
+// stays in registers; the top fluxes of the last right-hand side go to LDS
+// for the lanes that accumulate them.  *min_dt receives the smallest step
+// taken.  A launch stops after kAeFusedMaxAttempts attempts (errflag bit 2).
+template <int LPC, int ARITH>
+__global__ __launch_bounds__(kAeFusedBlock) void k_ae_fused(const Dev *dptr, const int *list, int nlist,
+                                                           const sedcol_params *__restrict__ pptr, int rec, double dt,
+                                                           int nsteps, int *errflag, double *min_dt)
+{
+  // domain and parameters through pointers to device copies: passed by
+  // value they would sit in scalar registers for the whole run and spill
+  const Dev &d = *dptr;
+  const sedcol_params &p = *pptr;
+  constexpr int G = kAeFusedBlock / LPC;
+  __shared__ double sbc[G][2 * NV];
+  __shared__ double sF[G][NV];
+  const int g = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const bool valid = g < nlist;
+  const int c = valid ? list[g] : 0;
+  Lane L;
+  Geo Gm;
+  long long cell = 0;
+  if (valid) {
+    setup_lane(d, c, l, L, Gm, cell);
+    load_forcing<ARITH>(d, p, c, rec, Gm, L, sbc[g]);
+  } else {
+    L.l = l;
+    L.K = 0;
+    L.active = false;
+  }
+  __syncthreads();
+  const bool act = valid && L.active;
+  const bool own = valid && l < NV;
+  double x[NV], r[NV];
+#pragma unroll
+  for (int n = 0; n < NV; ++n) x[n] = act ? d.conc[n * d.ncell + cell] : 0.0;
+  double sC = 0.0, sN = 0.0;
+  // sums touched only by accepted attempts live in LDS, leaving the
+  // registers to the right-hand side: the DIC and N2 sinks with their
+  // compensations, the lane's top-flux sum and compensation, and the
+  // column's diagnostics (updated by its top lane)
+  __shared__ double sacc[6][kAeFusedBlock];
+  __shared__ sedcol_coldiag sdg[G];
+  double(&acc)[6][kAeFusedBlock] = sacc;
+  const int t = threadIdx.x;
+  sedcol_coldiag &dg = sdg[g];
+  acc[0][t] = act ? d.cellacc[cell] : 0.0;
+  acc[1][t] = act ? d.cellacc[d.ncell + cell] : 0.0;
+  acc[2][t] = act ? d.cellacc[5 * d.ncell + cell] : 0.0;
+  acc[3][t] = act ? d.cellacc[6 * d.ncell + cell] : 0.0;
+  acc[4][t] = own ? d.colacc[(long long)l * d.ncol + c] : 0.0;
+  acc[5][t] = own ? d.colacc[(long long)(NV + l) * d.ncol + c] : 0.0;
+  if (valid && l == 0) dg = d.diag[c];
+  const double rcm = p.relative_change_min;
+  double mind = INFINITY;
+  int attempts = 0;
+  bool stop = false;
+  for (int s = 0; s < nsteps && !stop; ++s) {
+    double dt_int = 0.0, dt_red = dt;
+    bool need = true;
+    while (dt_int < dt) {
+      if (++attempts > kAeFusedMaxAttempts) {
+        if (threadIdx.x == 0) atomicOr(errflag, 2);
+        stop = true;
+        break;
+      }
+      if (need) {
+        if (valid) {
+          column_rhs<LPC, ARITH>(p, L, sbc[g], x, sC, sN, [&](int n, double rr, double F) {
+            r[n] = rr;
+            if (l == 0) {
+              sF[g][n] = F;
+              d.flux[(long long)n * d.ncol + c] = F;
+            }
+          }, &d.zw);
+        }
+        need = false;
+        __syncthreads();
+      }
+      bool v = false;
+      if (act) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+          const double c1 = x[n] + dt_red * r[n];
+          v |= (c1 - (1.0 + rcm) * x[n]) < 0.0;
+        }
+      }
+      const int viol = __syncthreads_or(v);
+      if (viol && dt_red > p.dt_min) {
+        dt_red = dt_red * 0.25;  // the state is unchanged: the right-hand side is reused
+      } else {
+        if (dt_red < mind) mind = dt_red;
+        if (own) kadd(acc[4][t], acc[5][t], dt_red * sF[g][l]);
+        if (valid && l == 0) {
+          dg.accepted += 1;
+          dg.dt_last = dt_red;
+          if (dt_red < dg.dt_minacc) dg.dt_minacc = dt_red;
+        }
+        if (act) {
+#pragma unroll
+          for (int n = 0; n < NV; ++n) x[n] = x[n] + dt_red * r[n];
+          kadd(acc[0][t], acc[2][t], dt_red * sC);
+          kadd(acc[1][t], acc[3][t], dt_red * sN);
+        }
+        dt_int = dt_int + dt_red;
+        need = true;
+        // the lanes reading sF are done before the next evaluation writes it
+        __syncthreads();
+      }
+    }
+    if (stop) break;
+    if (p.clamp && act) {
+      bool nan = false;
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        nan |= x[n] != x[n];
+        if (x[n] < 0.0) {
+          if (elem(n) >= 0)
+            kadd(d.cellacc[(2 + elem(n)) * d.ncell + cell], d.cellacc[(7 + elem(n)) * d.ncell + cell], 0.0 - x[n]);
+          x[n] = 0.0;
+        }
+      }
+      if (nan) atomicOr(errflag, 1);
+    }
+  }
+  if (act) {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) d.conc[n * d.ncell + cell] = x[n];
+    d.cellacc[cell] = acc[0][t];
+    d.cellacc[d.ncell + cell] = acc[1][t];
+    d.cellacc[5 * d.ncell + cell] = acc[2][t];
+    d.cellacc[6 * d.ncell + cell] = acc[3][t];
+  }
+  if (own) {
+    d.colacc[(long long)l * d.ncol + c] = acc[4][t];
+    d.colacc[(long long)(NV + l) * d.ncol + c] = acc[5][t];
+  }
+  if (valid && l == 0) d.diag[c] = dg;
+  if (threadIdx.x == 0) *min_dt = mind;
+}
+
 // minimum clamp after an ode_solver call (fabm_sediment_component.F90:1726-1731)
 template <int LPC>
 __global__ __launch_bounds__(BLOCK) void k_clamp(Dev d, const int *list, int nlist, int *errflag)
@@ -1938,6 +2085,14 @@ void launch_ae_commit(int lpc, hipStream_t st, const Dev &d, const int *list, in
 {
   SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_ae_commit<LPC>), grid_for<LPC>(n), dim3(BLOCK), 0, st, d, list, n,
                                           dt_red, r_in, s_in));
+}
+
+void launch_ae_fused(int lpc, hipStream_t st, const Dev *d_dev, const int *list, int n, const sedcol_params &p,
+                     const sedcol_params *d_params, int rec, double dt, int nsteps, int *errflag, double *min_dt)
+{
+  SC_DISPATCH_ARITH(p, SC_DISPATCH_LPC(lpc, hipLaunchKernelGGL((k_ae_fused<LPC, AR>), dim3(1), dim3(kAeFusedBlock),
+                                                                0, st, d_dev, list, n, d_params, rec, dt, nsteps,
+                                                                errflag, min_dt)));
 }
 
 void launch_clamp(int lpc, hipStream_t st, const Dev &d, const int *list, int n, int *errflag)

@@ -109,6 +109,15 @@ void launch_ae_check(int lpc, hipStream_t st, const Dev &d, const int *list, int
                      const double *r_in, int *viol);
 void launch_ae_commit(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double dt_red,
                       const double *r_in, const double *s_in);
+// ADAPTIVE_EULER steps of a domain whose wet columns all sit in one bin of
+// at most kAeFusedBlock / lpc columns, in one workgroup (no host round trip
+// per attempt); errflag bit 1 NaN, bit 2 attempt limit; *min_dt (device) the
+// smallest step taken
+constexpr int kAeFusedBlock = 512;
+constexpr int kAeFusedMaxAttempts = 1 << 22;
+// (d_dev, d_params: device copies of the domain view and of p)
+void launch_ae_fused(int lpc, hipStream_t st, const Dev *d_dev, const int *list, int n, const sedcol_params &p,
+                     const sedcol_params *d_params, int rec, double dt, int nsteps, int *errflag, double *min_dt);
 void launch_clamp(int lpc, hipStream_t st, const Dev &d, const int *list, int n, int *errflag);
 void launch_budget(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double *budget);
 void launch_inventory(int lpc, hipStream_t st, const Dev &d, const int *list, int n, double *inv);

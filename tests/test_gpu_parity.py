@@ -141,10 +141,14 @@ def test_rk4_windows_equal_single_launch(gpu_ready):
     assert np.array_equal(a, b)
 
 
-def test_adaptive_euler_global_rule(gpu_ready):
+@pytest.mark.parametrize("nij", [pytest.param(2, id="one_workgroup"), pytest.param(6, id="multi_launch")])
+def test_adaptive_euler_global_rule(gpu_ready, nij):
     """ADAPTIVE_EULER quarters the step for every column when any cell
-    violates the relative-change test (solver_library.F90:119-129)."""
-    dom = cases.plain_columns(ni=2, nj=2, K=12, seed=7)
+    violates the relative-change test (solver_library.F90:119-129).  4
+    columns run in one workgroup (the domain-wide vote inside the kernel),
+    36 (more than one workgroup of 16-lane groups holds) in one launch per
+    stage with the vote read by the host; both bitwise like the oracle."""
+    dom = cases.plain_columns(ni=nij, nj=nij, K=12, seed=7)
     dom.params.dt_min = 1.0
     p = dom.params.si()
     x0 = dom.conc0.copy()
