@@ -14,7 +14,8 @@
  *     sedcol_status code; sedcol_last_error() describes the last failure of
  *     the calling thread.  Nothing calls exit/stop/abort.
  *   - all pointers are caller-owned host memory, read or written during the
- *     call only (the library copies); FP64 everywhere.
+ *     call only (the library copies), except the profiles given to
+ *     sedcol_set_state_profiles (device memory); FP64 everywhere.
  *   - array layout is chosen at create time (sedcol_desc.layout):
  *       SEDCOL_LAYOUT_FORTRAN  uniform K; cell arrays (ncol, K) and state
  *                              (ncol, K, nvar) with the column index fastest,
