@@ -69,6 +69,20 @@ def test_descriptor_errors_need_no_device():
             raise AssertionError(f"layout {layout} with nlayers {nl.tolist()} accepted")
 
 
+def test_warm_start_entry_points_check_arguments_without_a_device():
+    """sedcol_set_state_profiles and sedsurr_predict_device reject a null
+    handle before any HIP call."""
+    import ctypes
+
+    from sedcol import surrogate
+
+    rc = capi.lib().sedcol_set_state_profiles(None, None, 330, 30, None, None, None, 1e-3, None)
+    assert rc == capi.SEDCOL_EINVAL
+    out = ctypes.c_void_p()
+    rc = surrogate.lib().sedsurr_predict_device(None, None, 1, ctypes.byref(out))
+    assert rc == capi.SEDCOL_EINVAL and not out.value
+
+
 def test_bench_refuses_more_than_one_gpu():
     """bench.py measures one GPU; asking for more exits with status 2 and
     says so, before touching any device."""
