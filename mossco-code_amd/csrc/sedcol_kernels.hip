@@ -895,10 +895,14 @@ struct RosShared {
   double G[2][LPC];     // transport coefficients D/dzc (D/dz at the top): [0] particulate, [1] dissolved
   double W[LPC];        // 1/(phi dz)
   double ph[LPC];       // porosity
-  double T[NV][LPC];    // right-hand sides and solutions of the species sweeps
-  double inv[NV][LPC];  // reciprocal Thomas pivots
-  double A[NV][LPC];    // sub-diagonal couplings of the forward sweep
-  double E[NV][LPC];    // super-diagonal coupling times reciprocal pivot (back substitution)
+  // Species rows padded by one double: the sweep of species n runs in lane
+  // n, so the lanes read one layer of 11 different rows at once; with the
+  // padding those sit in different LDS banks (unpadded, the row pitch is a
+  // multiple of the bank count and the 11 reads serialise)
+  double T[NV][LPC + 1];    // right-hand sides and solutions of the species sweeps
+  double inv[NV][LPC + 1];  // reciprocal Thomas pivots
+  double A[NV][LPC + 1];    // sub-diagonal couplings of the forward sweep
+  double E[NV][LPC + 1];    // super-diagonal coupling times reciprocal pivot (back substitution)
   double top[2][NV];    // linearised Dirichlet top-flux terms of the two stages
 };
 
