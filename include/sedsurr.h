@@ -75,8 +75,8 @@ int sedsurr_loss_grad(sedsurr_handle *h, const float *X, const float *Y, int32_t
 int sedsurr_predict(sedsurr_handle *h, const float *X, int64_t n, float *Y);
 /* The same with Y left on the GPU, for example as the input of
  * sedcol_set_state_profiles, so that a warm start never passes through host
- * memory.  X is host memory; *Y_device receives the address of a device
- * buffer of the handle (n x n_out, row-major) that stays valid until the
+ * memory.  X is host memory, n >= 1; *Y_device receives the address of a
+ * device buffer of the handle (n x n_out, row-major) that stays valid until the
  * next sedsurr_predict_device call or sedsurr_destroy.  The buffer grows to
  * the largest n asked for and is freed with the handle. */
 int sedsurr_predict_device(sedsurr_handle *h, const float *X, int64_t n, const float **Y_device);
